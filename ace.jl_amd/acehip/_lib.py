@@ -1,0 +1,158 @@
+"""ctypes binding of libacehip.so (the C ABI declared in include/acehip.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if the library is missing or fails to load, every evaluation raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libacehip.so")
+
+ABI_VERSION = 1
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class AcePack(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("prop", C.c_int32), ("L", C.c_int32), ("nr", C.c_int32),
+        ("nspecies", C.c_int32), ("tp", C.c_int32),
+        ("r0", C.c_double), ("tl", C.c_double), ("tr", C.c_double),
+        ("pl", C.c_int32), ("pr", C.c_int32),
+        ("rA", _f64p), ("rB", _f64p), ("rC", _f64p),
+        ("npool", C.c_int32), ("pool", _i32p),
+        ("nA", C.c_int32), ("aspec_fid", _i32p), ("aspec_sgn", _f64p),
+        ("nAA", C.c_int32), ("maxorder", C.c_int32), ("orders", _i32p), ("iAA2iA", _i32p),
+        ("nB", C.c_int32), ("a2b_nnz", C.c_int32), ("a2b_rowptr", _i32p), ("a2b_cols", _i32p),
+        ("a2b_vals", _f64p),
+        ("nruns", C.c_int32), ("runs", _i32p),
+        ("nleaves", C.c_int32), ("leaf_fid", _i32p), ("leaf_term_ptr", _i32p),
+        ("ntasks", C.c_int32), ("tasks", _i32p), ("task_term_ptr", _i32p),
+        ("nterms", C.c_int32), ("term_row", _i32p), ("term_scale", _f64p), ("term_conj", _i32p),
+        ("ngslots", C.c_int32), ("gslot_ptr", _i32p), ("gslot_list", _i32p),
+        ("nwave_cfg", C.c_int32), ("wave_cfg_W", _i32p), ("wave_task_ptr", _i32p),
+        ("wave_task_list", _i32p), ("wave_etask_ptr", _i32p), ("wave_etask_list", _i32p),
+    ]
+
+
+# every symbol declared in include/acehip.h
+EXPORTS = ("ace_ctx_create", "ace_set_params", "ace_set_tile", "ace_energy",
+           "ace_energy_forces", "ace_basis_A", "ace_basis_AA", "ace_basis_B",
+           "ace_basis_dB", "ace_last_timing", "ace_set_validation", "ace_last_error",
+           "ace_ctx_destroy", "ace_abi_version")
+
+_lib = None
+
+
+class AceError(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load libacehip.so (raises if it is missing: no CPU fallback exists)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise AceError(f"libacehip.so not built ({p}); run __graft_entry__.build()")
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    lib.ace_ctx_create.argtypes = [C.POINTER(AcePack), C.c_int, _f64p, C.c_int64, C.POINTER(vp)]
+    lib.ace_set_params.argtypes = [vp, _f64p, C.c_int64]
+    lib.ace_set_tile.argtypes = [vp, C.c_int]
+    ev = [vp, C.c_int64, vp, vp, vp, vp, vp]
+    lib.ace_energy.argtypes = ev
+    lib.ace_energy_forces.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
+    for name in ("ace_basis_A", "ace_basis_AA", "ace_basis_B", "ace_basis_dB"):
+        getattr(lib, name).argtypes = ev
+    lib.ace_last_timing.argtypes = [vp, _f64p, _f64p, _f64p]
+    lib.ace_set_validation.argtypes = [vp, C.c_int]
+    lib.ace_last_error.restype = C.c_char_p
+    lib.ace_last_error.argtypes = []
+    lib.ace_ctx_destroy.argtypes = [vp]
+    lib.ace_ctx_destroy.restype = None
+    lib.ace_abi_version.restype = C.c_int
+    if lib.ace_abi_version() != ABI_VERSION:
+        raise AceError("libacehip.so ABI version mismatch")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = load().ace_last_error().decode()
+        raise AceError(msg)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def make_pack(pk):
+    """Fill an AcePack from a :class:`acehip.pack.Pack`; returns (struct,
+    keepalive list)."""
+    T = pk.T
+    rn = T["rn"]
+    keep = []
+
+    def arr(x, dt):
+        a = np.ascontiguousarray(x, dtype=dt)
+        keep.append(a)
+        return a
+
+    s = AcePack()
+    s.abi_version = ABI_VERSION
+    s.prop = pk.prop
+    s.L = int(T["L"])
+    s.nr = len(rn["A"])
+    s.nspecies = pk.nspecies
+    s.tp = int(rn["p"])
+    s.r0, s.tl, s.tr = float(rn["r0"]), float(rn["tl"]), float(rn["tr"])
+    s.pl, s.pr = int(rn["pl"]), int(rn["pr"])
+    s.rA = _ptr(arr(rn["A"], np.float64), C.c_double)
+    s.rB = _ptr(arr(rn["B"], np.float64), C.c_double)
+    s.rC = _ptr(arr(rn["C"], np.float64), C.c_double)
+    s.npool = len(pk.pool)
+    s.pool = _ptr(arr(pk.pool, np.int32), C.c_int32)
+    s.nA = len(pk.aspec_fid)
+    s.aspec_fid = _ptr(arr(pk.aspec_fid, np.int32), C.c_int32)
+    s.aspec_sgn = _ptr(arr(pk.aspec_sgn, np.float64), C.c_double)
+    s.nAA = len(pk.orders)
+    s.maxorder = pk.maxorder
+    s.orders = _ptr(arr(pk.orders, np.int32), C.c_int32)
+    s.iAA2iA = _ptr(arr(pk.iAA2iA, np.int32), C.c_int32)
+    s.nB = pk.nB
+    s.a2b_nnz = len(pk.a2b_cols)
+    s.a2b_rowptr = _ptr(arr(pk.a2b_rowptr, np.int32), C.c_int32)
+    s.a2b_cols = _ptr(arr(pk.a2b_cols, np.int32), C.c_int32)
+    s.a2b_vals = _ptr(arr(pk.a2b_vals, np.float64), C.c_double)
+    s.nruns = len(pk.runs)
+    s.runs = _ptr(arr(pk.runs, np.int32), C.c_int32)
+    s.nleaves = len(pk.leaf_fid)
+    s.leaf_fid = _ptr(arr(pk.leaf_fid, np.int32), C.c_int32)
+    s.leaf_term_ptr = _ptr(arr(pk.leaf_term_ptr, np.int32), C.c_int32)
+    s.ntasks = len(pk.tasks)
+    s.tasks = _ptr(arr(pk.tasks, np.int32), C.c_int32)
+    s.task_term_ptr = _ptr(arr(pk.task_term_ptr, np.int32), C.c_int32)
+    s.nterms = len(pk.term_row)
+    s.term_row = _ptr(arr(pk.term_row, np.int32), C.c_int32)
+    s.term_scale = _ptr(arr(pk.term_scale, np.float64), C.c_double)
+    s.term_conj = _ptr(arr(pk.term_conj, np.int32), C.c_int32)
+    s.ngslots = pk.ngslots
+    s.gslot_ptr = _ptr(arr(pk.gslot_ptr, np.int32), C.c_int32)
+    s.gslot_list = _ptr(arr(pk.gslot_list, np.int32), C.c_int32)
+    s.nwave_cfg = len(pk.wave_cfg_W)
+    s.wave_cfg_W = _ptr(arr(pk.wave_cfg_W, np.int32), C.c_int32)
+    s.wave_task_ptr = _ptr(arr(pk.wave_task_ptr, np.int32), C.c_int32)
+    s.wave_task_list = _ptr(arr(pk.wave_task_list, np.int32), C.c_int32)
+    s.wave_etask_ptr = _ptr(arr(pk.wave_etask_ptr, np.int32), C.c_int32)
+    s.wave_etask_list = _ptr(arr(pk.wave_etask_list, np.int32), C.c_int32)
+    return s, keep

@@ -1,0 +1,229 @@
+"""Device evaluator and the ACE.jl-shaped host API.
+
+``HIPEvaluator`` occupies the evaluator slot of ACE.jl's ``LinearACEModel``
+(linearmodel.jl:44-56, 107-111): ``evaluate(model, cfg)`` and
+``grad_config(model, cfg)`` dispatch to it, ``set_params!`` forwards new
+coefficients (linearmodel.jl:67-71).  Basis-level ``evaluate`` /
+``evaluate_d`` of Product1pBasis, PIBasis and SymmetricBasis run on the same
+context.  Batches of environments are the native unit; a single
+configuration is a batch of one.
+
+Device memory and streams come from PyTorch; every computation runs in
+libacehip.so.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .pack import Pack
+from .tables import basis_tables
+
+
+class EnvBatch:
+    """Ragged batch of environments resident on the device.
+
+    ``offsets[nenv+1]`` int32, ``xyz[3, nnbr]`` float64 SoA, ``species[nnbr]``
+    int8 (category index) or None.
+    """
+
+    def __init__(self, offsets, xyz, species=None, device="cuda"):
+        self.offsets = torch.as_tensor(np.asarray(offsets, dtype=np.int32) if not torch.is_tensor(offsets) else offsets,
+                                       dtype=torch.int32, device=device).contiguous()
+        x = xyz if torch.is_tensor(xyz) else torch.as_tensor(np.asarray(xyz, dtype=np.float64))
+        x = x.to(device=device, dtype=torch.float64)
+        if x.dim() != 2 or x.shape[0] != 3:
+            raise ValueError("xyz must be SoA with shape (3, nnbr)")
+        self.xyz = x.contiguous()
+        self.species = None
+        if species is not None:
+            s = species if torch.is_tensor(species) else torch.as_tensor(np.asarray(species, dtype=np.int8))
+            self.species = s.to(device=device, dtype=torch.int8).contiguous()
+        self.nenv = self.offsets.numel() - 1
+        self.nnbr = self.xyz.shape[1]
+
+    @classmethod
+    def from_configs(cls, cfgs, species=None, device="cuda"):
+        """``cfgs``: list of (nX, 3) arrays; ``species``: list of (nX,) ints."""
+        lens = [len(c) for c in cfgs]
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        R = np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1, 3) for c in cfgs], axis=0) \
+            if cfgs else np.zeros((0, 3))
+        sp = None
+        if species is not None:
+            sp = np.concatenate([np.asarray(s, dtype=np.int8) for s in species]) if cfgs else np.zeros(0, np.int8)
+        return cls(off, R.T.copy(), sp, device=device)
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class HIPEvaluator:
+    """Context of libacehip.so for one SymmetricBasis and coefficients."""
+
+    def __init__(self, basis, c, device: int = 0, waves: int = 8):
+        self.lib = _lib.load()
+        self.basis = basis
+        self.T = basis_tables(basis)
+        self.pack = Pack(self.T)
+        self.prop = self.pack.prop
+        self.device = device
+        struct, keep = _lib.make_pack(self.pack)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        ctx = C.c_void_p()
+        _lib.check(self.lib.ace_ctx_create(C.byref(struct), device, c.ctypes.data_as(_lib._f64p),
+                                           len(c), C.byref(ctx)))
+        self.ctx = ctx
+        self.set_tile(waves)
+
+    def __del__(self):
+        ctx = getattr(self, "ctx", None)
+        if ctx:
+            self.lib.ace_ctx_destroy(ctx)
+            self.ctx = None
+
+    def set_tile(self, waves: int):
+        _lib.check(self.lib.ace_set_tile(self.ctx, int(waves)))
+        self.waves = int(waves)
+
+    def set_validation(self, on: bool):
+        _lib.check(self.lib.ace_set_validation(self.ctx, int(bool(on))))
+
+    def set_params(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        _lib.check(self.lib.ace_set_params(self.ctx, c.ctypes.data_as(_lib._f64p), len(c)))
+
+    def _stream(self, stream):
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        return C.c_void_p(stream.cuda_stream)
+
+    # -- model level
+    def energy(self, b: EnvBatch, out=None, stream=None):
+        nc = 1 if self.prop == 0 else 3
+        if out is None:
+            out = torch.empty((b.nenv,) if nc == 1 else (b.nenv, 3), dtype=torch.float64,
+                              device=b.xyz.device)
+        _lib.check(self.lib.ace_energy(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
+                                       _vp(out), self._stream(stream)))
+        return out
+
+    def energy_forces(self, b: EnvBatch, E=None, dE=None, stream=None):
+        nc = 1 if self.prop == 0 else 3
+        if E is None:
+            E = torch.empty((b.nenv,) if nc == 1 else (b.nenv, 3), dtype=torch.float64,
+                            device=b.xyz.device)
+        if dE is None:
+            dE = torch.empty((3 * nc, b.nnbr), dtype=torch.float64, device=b.xyz.device)
+        _lib.check(self.lib.ace_energy_forces(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz),
+                                              _vp(b.species), _vp(E), _vp(dE), self._stream(stream)))
+        return E, dE
+
+    def last_timing(self):
+        a, b_, c = C.c_double(), C.c_double(), C.c_double()
+        _lib.check(self.lib.ace_last_timing(self.ctx, C.byref(a), C.byref(b_), C.byref(c)))
+        return {"dag_ms": a.value, "pool_ms": b_.value, "force_ms": c.value}
+
+    # -- basis level
+    def basis_A(self, b: EnvBatch, stream=None):
+        out = torch.empty((b.nenv, self.pack.aspec_fid.size, 2), dtype=torch.float64, device=b.xyz.device)
+        _lib.check(self.lib.ace_basis_A(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
+                                        _vp(out), self._stream(stream)))
+        return torch.view_as_complex(out)
+
+    def basis_AA(self, b: EnvBatch, stream=None):
+        nAA = len(self.pack.orders)
+        if self.prop == 0:
+            out = torch.empty((b.nenv, nAA), dtype=torch.float64, device=b.xyz.device)
+        else:
+            out = torch.empty((b.nenv, nAA, 2), dtype=torch.float64, device=b.xyz.device)
+        _lib.check(self.lib.ace_basis_AA(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
+                                         _vp(out), self._stream(stream)))
+        return out if self.prop == 0 else torch.view_as_complex(out)
+
+    def basis_B(self, b: EnvBatch, stream=None):
+        shape = (b.nenv, self.pack.nB) if self.prop == 0 else (b.nenv, self.pack.nB, 3)
+        out = torch.empty(shape, dtype=torch.float64, device=b.xyz.device)
+        _lib.check(self.lib.ace_basis_B(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
+                                        _vp(out), self._stream(stream)))
+        return out
+
+    def basis_dB(self, b: EnvBatch, stream=None):
+        shape = (b.nnbr, self.pack.nB, 3) if self.prop == 0 else (b.nnbr, self.pack.nB, 3, 3)
+        out = torch.zeros(shape, dtype=torch.float64, device=b.xyz.device)
+        _lib.check(self.lib.ace_basis_dB(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
+                                         _vp(out), self._stream(stream)))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# ACE.jl-shaped functional API
+
+
+def _as_batch(cfg, species=None):
+    if isinstance(cfg, EnvBatch):
+        return cfg, False
+    arr = np.asarray(cfg, dtype=np.float64)
+    if arr.ndim == 2 and arr.shape[1] == 3:
+        return EnvBatch.from_configs([arr], None if species is None else [species]), True
+    raise TypeError("cfg must be an (nX, 3) array of neighbour positions or an EnvBatch")
+
+
+def _ev(model):
+    if getattr(model, "_ev", None) is None:
+        model._ev = HIPEvaluator(model.basis, model.c)
+    return model._ev
+
+
+def evaluate(obj, cfg, species=None):
+    """``evaluate(model, cfg)`` -> site energy (linearmodel.jl:107-108);
+    ``evaluate(basis::SymmetricBasis, cfg)`` -> B (symmbasis.jl:297-302);
+    PIBasis -> AA; Product1pBasis -> A."""
+    from .symmbasis import LinearACEModel, SymmetricBasis
+    from .pibasis import PIBasis
+
+    b, single = _as_batch(cfg, species)
+    if isinstance(obj, LinearACEModel):
+        E = _ev(obj).energy(b).cpu().numpy()
+        return E[0] if single else E
+    if isinstance(obj, SymmetricBasis):
+        ev = _basis_ev(obj)
+        B = ev.basis_B(b).cpu().numpy()
+        return B[0] if single else B
+    if isinstance(obj, PIBasis):
+        ev = _basis_ev(obj._symm)
+        AA = ev.basis_AA(b).cpu().numpy()
+        return AA[0] if single else AA
+    raise TypeError("evaluate: unsupported object")
+
+
+def _basis_ev(basis):
+    if getattr(basis, "_ev", None) is None:
+        basis._ev = HIPEvaluator(basis, np.zeros(len(basis)))
+    return basis._ev
+
+
+def grad_config(model, cfg, species=None):
+    """``grad_config(model, cfg)``: +dE/dr_j per neighbour (evaluator.jl:150-200)."""
+    b, single = _as_batch(cfg, species)
+    _, dE = _ev(model).energy_forces(b)
+    g = dE.cpu().numpy()
+    if model._ev.prop == 0:
+        return g.T.copy()
+    return g.reshape(3, 3, -1).transpose(2, 0, 1).copy()
+
+
+def grad_params(model, cfg, species=None):
+    """``grad_params`` = B (linearmodel.jl:114-123)."""
+    return evaluate(model.basis, cfg, species)
+
+
+def evaluate_d(basis, cfg, species=None):
+    """``evaluate_d(basis::SymmetricBasis, cfg)``: dB[nX, nB, 3]
+    (symmbasis.jl:322-336)."""
+    b, _ = _as_batch(cfg, species)
+    return _basis_ev(basis).basis_dB(b).cpu().numpy()

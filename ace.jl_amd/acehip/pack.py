@@ -1,0 +1,139 @@
+"""Device basis pack: the flat arrays handed to ``ace_ctx_create``.
+
+Built once per basis on the CPU from the shared tables
+(:func:`acehip.tables.basis_tables`).  Holds the radial recursion, the
+pooled ``m >= 0`` table, the AA / A2B tables for the basis-level calls, and
+the run programs of the energy and of the gradient polynomials together with
+their task -> wave assignments for every supported tile shape.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import programs as P
+
+WAVE_CONFIGS = (4, 8, 16)     # waves per workgroup of the product-DAG kernel
+ENERGY_PARTS = 16             # energy polynomial split into this many tasks
+MAX_TASK_COST = 6000.0        # gradient polynomials above this are split
+
+
+class Pack:
+    def __init__(self, T: dict):
+        self.T = T
+        self.prop = 0 if T["prop"] == "Invariant" else 1
+        aspec = np.asarray(T["aspec"], dtype=np.int64)
+        self.pool, kk, cj, sgn = P.pool_map(aspec)
+        self.aspec_fid = (kk | (cj * P.DEV_CONJ)).astype(np.int32)
+        self.aspec_sgn = sgn.astype(np.float64)
+        self.nspecies = 0 if T["categories"] is None else len(T["categories"])
+        orders = np.asarray(T["orders"], dtype=np.int64)
+        iAA2iA = np.asarray(T["iAA2iA"], dtype=np.int64)
+        if iAA2iA.shape[1] > 4:
+            raise ValueError("correlation order above 4 is not supported")
+        poly, _ = P.energy_polynomial(orders, iAA2iA, kk, cj, sgn)
+        self.poly = poly
+        is_m0 = self.pool[:, 3] == 0
+        G = P.gradient_polynomials(poly, len(self.pool), is_m0)
+        prog = P.RunProgram()
+        self.energy_tasks = [prog.add_task(part, -1) for part in P.split_poly(poly, ENERGY_PARTS)]
+        # gradient tasks: split large polynomials over several output slots
+        gslots = [[] for _ in range(len(self.pool))]
+        self.grad_tasks = []
+        nslot = 0
+        for k, g in enumerate(G):
+            if not g:
+                continue
+            cost = sum(1.0 for m in g) + 0.0
+            nparts = max(1, int(np.ceil(cost / MAX_TASK_COST)))
+            parts = P.split_poly(g, nparts) if nparts > 1 else [g]
+            for part in parts:
+                t = prog.add_task(part, nslot)
+                self.grad_tasks.append(t)
+                gslots[k].append(nslot)
+                nslot += 1
+        self.ngslots = nslot
+        self.gslot_ptr = np.zeros(len(self.pool) + 1, dtype=np.int32)
+        self.gslot_ptr[1:] = np.cumsum([len(s) for s in gslots])
+        self.gslot_list = np.asarray([s for ss in gslots for s in ss], dtype=np.int32)
+        self.prog = prog
+        # flatten programs
+        self.runs = np.asarray(prog.runs, dtype=np.int32).reshape(-1, 7)
+        self.leaf_fid = np.asarray(prog.leaf_fid, dtype=np.int32)
+        self.tasks = np.asarray(prog.tasks, dtype=np.int32).reshape(-1, 3)
+        lt_ptr, rows, scales, conjs = [0], [], [], []
+        for terms in prog.leaf_terms:
+            for (r, s, c) in terms:
+                rows.append(r)
+                scales.append(s)
+                conjs.append(c)
+            lt_ptr.append(len(rows))
+        tt_ptr = [len(rows)]
+        for terms in prog.task_terms:
+            for (r, s, c) in terms:
+                rows.append(r)
+                scales.append(s)
+                conjs.append(c)
+            tt_ptr.append(len(rows))
+        # task terms follow the leaf terms in the same arrays
+        self.leaf_term_ptr = np.asarray(lt_ptr, dtype=np.int32)
+        self.task_term_ptr = np.asarray(tt_ptr, dtype=np.int32)
+        self.term_row = np.asarray(rows, dtype=np.int32)
+        self.term_scale = np.asarray(scales, dtype=np.float64)
+        self.term_conj = np.asarray(conjs, dtype=np.int32)
+        # wave assignments
+        costs = [prog.cost(t) for t in range(len(prog.tasks))]
+        self.wave_cfg_W = np.asarray(WAVE_CONFIGS, dtype=np.int32)
+        wtp, wtl, wep, wel = [], [], [], []
+        self.wave_load = {}
+        for W in WAVE_CONFIGS:
+            lists, load = P.lpt_assign(costs, W)
+            wtp.extend(np.concatenate([[0], np.cumsum([len(l) for l in lists])]).tolist())
+            for l in lists:
+                wtl.extend(l)
+            ecost = [costs[t] for t in self.energy_tasks]
+            elists, eload = P.lpt_assign(ecost, W)
+            elists = [[self.energy_tasks[i] for i in l] for l in elists]
+            wep.extend(np.concatenate([[0], np.cumsum([len(l) for l in elists])]).tolist())
+            for l in elists:
+                wel.extend(l)
+            self.wave_load[W] = (max(load), sum(load), max(eload), sum(eload))
+        self.wave_task_ptr = np.asarray(wtp, dtype=np.int32)
+        self.wave_task_list = np.asarray(wtl, dtype=np.int32)
+        self.wave_etask_ptr = np.asarray(wep, dtype=np.int32)
+        self.wave_etask_list = np.asarray(wel, dtype=np.int32)
+        # A2B CSR
+        a2b = T["a2b"]
+        rows_ = np.asarray(a2b["rows"], dtype=np.int64)
+        cols_ = np.asarray(a2b["cols"], dtype=np.int64)
+        vals_ = np.asarray(a2b["vals"])
+        order = np.lexsort((cols_, rows_))
+        rows_, cols_, vals_ = rows_[order], cols_[order], vals_[order]
+        nB, nAA = a2b["shape"]
+        self.nB, self.nAA = int(nB), int(nAA)
+        rp = np.zeros(self.nB + 1, dtype=np.int64)
+        np.add.at(rp, rows_ + 1, 1)
+        self.a2b_rowptr = np.cumsum(rp).astype(np.int32)
+        self.a2b_cols = cols_.astype(np.int32)
+        if self.prop == 0:
+            self.a2b_vals = np.asarray(vals_, dtype=np.float64)
+        else:
+            v = np.asarray(vals_, dtype=np.complex128).reshape(-1, 3)
+            self.a2b_vals = np.stack([v.real, v.imag], axis=-1).reshape(-1)
+        self.orders = orders.astype(np.int32)
+        self.maxorder = int(iAA2iA.shape[1])
+        self.iAA2iA = iAA2iA.astype(np.int32)
+
+    # -- statistics used by the roofline accounting
+    def work_counts(self, forces: bool):
+        """Executed leaf and run counts per environment of the product-DAG
+        kernel (energy tasks, plus gradient tasks with forces)."""
+        tasks = list(self.energy_tasks) + (list(self.grad_tasks) if forces else [])
+        leaves = runs = 0
+        newf = 0
+        for t in tasks:
+            r0, r1, _ = self.tasks[t]
+            rr = self.runs[r0:r1]
+            leaves += int(rr[:, 5].sum())
+            runs += len(rr)
+            newf += int(np.sum(np.maximum(rr[:, 0] - rr[:, 1] + 1, 0)))
+        return {"leaves": leaves, "runs": runs, "prefix_mults": newf}

@@ -1,0 +1,203 @@
+"""Compile the AA contraction into trie "run programs" for the device.
+
+The energy of an environment is a polynomial in the 1p values,
+``P(A) = c0 + sum_r c~_r prod_t A[i_rt]`` (evaluator.jl:121-147), with every
+1p value ``A[iA] = s * conj?(A+[k])`` expressed through the pooled ``m >= 0``
+values ``A+`` (``Y_l^{-m} = (-1)^m conj(Y_l^m)``, sphericalharmonics.jl:
+394-398, and R_n real).
+
+Monomials are grouped by their prefix (all factors but the last, factors in
+ascending order of the pooled index, which is sorted by polynomial degree);
+one *run* is a prefix plus its *leaves* (last factor + coefficient), so
+``P = c0 + sum_runs Pprefix * sum_leaves c * A[leaf]``.  Runs are sorted
+lexicographically so consecutive runs share leading prefix factors, which the
+kernel keeps as running products in registers (depth-first evaluation of the
+product DAG).
+
+The forces need ``G[k] = dP/dA+[k] + conj(dP/dconj(A+[k]))`` for every
+pooled value k: the adjoint of ``E = Re P`` (evaluator.jl:174-192) restated
+in terms of A+, because ``dE = Re sum_k G[k] dA+[k]``.  Each ``G[k]`` is again
+a polynomial (one order less) and is compiled the same way.
+
+Coefficients are kept symbolically as *terms* ``(AA row, scale, conj)`` with
+value ``scale * c~[row]`` (conjugated when ``conj``), so that new model
+parameters only need ``c~ = A2B^T c`` and a gather (``ace_set_params``).
+
+Internal factor id: ``2k + conj``; device factor id: ``k | 0x8000*conj``.
+"""
+from __future__ import annotations
+
+from collections import defaultdict
+
+import numpy as np
+
+DEV_CONJ = 0x8000
+
+
+def pool_map(aspec):
+    """Pooled ``m >= 0`` table and the map A index -> (k, conj, sign).
+
+    Returns ``(pool, kk, cj, sgn)``: ``pool[k] = (mu, n, l, m>=0)`` sorted by
+    (degree n-1+l, mu, l, m, n); for every 1p function ``iA``: pooled index
+    ``kk[iA]``, ``cj[iA] = m < 0`` and ``sgn[iA] = (-1)^m`` for ``m < 0``.
+    """
+    keys = sorted({(int(mu), int(n), int(l), abs(int(m))) for (mu, n, l, m) in aspec},
+                  key=lambda p: (p[1] - 1 + p[2], p[0], p[2], p[3], p[1]))
+    index = {p: k for k, p in enumerate(keys)}
+    kk = np.zeros(len(aspec), dtype=np.int64)
+    cj = np.zeros(len(aspec), dtype=np.int64)
+    sgn = np.ones(len(aspec))
+    for i, (mu, n, l, m) in enumerate(aspec):
+        kk[i] = index[(int(mu), int(n), int(l), abs(int(m)))]
+        if m < 0:
+            cj[i] = 1
+            sgn[i] = -1.0 if (m % 2) else 1.0
+    return np.asarray(keys, dtype=np.int64).reshape(-1, 4), kk, cj, sgn
+
+
+def energy_polynomial(orders, iAA2iA, kk, cj, sgn):
+    """Monomials of P with symbolic coefficients.
+
+    Returns ``(poly, const_terms)``: ``poly[mono] = [(row, scale, conj)]``
+    with ``mono`` a tuple of internal factor ids in ascending order."""
+    poly = {}
+    const_terms = []
+    for i in range(len(orders)):
+        o = int(orders[i])
+        if o == 0:
+            const_terms.append((i, 1.0, 0))
+            continue
+        idx = iAA2iA[i, :o]
+        s = float(np.prod(sgn[idx]))
+        mono = tuple(sorted(int(2 * kk[a] + cj[a]) for a in idx))
+        poly.setdefault(mono, []).append((i, s, 0))
+    return poly, const_terms
+
+
+def gradient_polynomials(poly, npool, is_m0):
+    """``G[k]`` polynomials (list over k) with symbolic coefficients.
+
+    The derivative w.r.t. ``A+[k]`` (factor ``2k``) is taken directly; the
+    derivative w.r.t. ``conj(A+[k])`` (factor ``2k+1``) is conjugated, which
+    maps every remaining factor to its conjugate partner and conjugates the
+    coefficient."""
+
+    def conj_id(x):
+        k = x >> 1
+        return 2 * k if is_m0[k] else (x ^ 1)
+
+    G = [defaultdict(list) for _ in range(npool)]
+    for mono, terms in poly.items():
+        seen = set()
+        for f in mono:
+            if f in seen:
+                continue
+            seen.add(f)
+            mult = mono.count(f)
+            rest = list(mono)
+            rest.remove(f)
+            k = f >> 1
+            if (f & 1) == 0:
+                key = tuple(sorted(rest))
+                for (row, s, c) in terms:
+                    G[k][key].append((row, s * mult, c))
+            else:
+                key = tuple(sorted(conj_id(x) for x in rest))
+                for (row, s, c) in terms:
+                    G[k][key].append((row, s * mult, 1 - c))
+    return [dict(g) for g in G]
+
+
+def eval_terms(terms, ct):
+    """Numeric value of a symbolic coefficient for c~ (real or (nAA,3))."""
+    v = 0.0
+    for (row, s, c) in terms:
+        x = ct[row] * s
+        v = v + (np.conj(x) if c else x)
+    return v
+
+
+class RunProgram:
+    """Flat encoding of a set of polynomials ("tasks").
+
+    run  = (depth, jstart, f1, f2, f3, nleaf, leaf_off): depth = prefix length
+           (0..3); jstart = first prefix level (1-based) that differs from the
+           previous run of the same task (depth+1 when none).
+    leaf = device factor id + symbolic coefficient terms.
+    task = (run_begin, run_end, out_slot) + constant terms; out_slot -1 is
+           the (real part of the) energy, otherwise a G slot.
+    """
+
+    def __init__(self):
+        self.runs = []
+        self.leaf_fid = []
+        self.leaf_terms = []
+        self.tasks = []
+        self.task_terms = []
+
+    @staticmethod
+    def dev(f):
+        return (f >> 1) | (DEV_CONJ if (f & 1) else 0)
+
+    def add_task(self, poly, out_slot, const_terms=()):
+        const_terms = list(const_terms)
+        groups = defaultdict(list)
+        for mono, terms in poly.items():
+            if len(mono) == 0:
+                const_terms.extend(terms)
+                continue
+            if len(mono) > 4:
+                raise ValueError("correlation order above 4 is not supported by the device programs")
+            groups[mono[:-1]].append((mono[-1], terms))
+        r0 = len(self.runs)
+        prev = ()
+        for pre in sorted(groups):
+            leaves = sorted(groups[pre], key=lambda x: x[0])
+            d = len(pre)
+            j = 1
+            while j <= d and j <= len(prev) and pre[j - 1] == prev[j - 1]:
+                j += 1
+            f = [self.dev(x) for x in pre] + [0] * (3 - d)
+            self.runs.append((d, j, f[0], f[1], f[2], len(leaves), len(self.leaf_fid)))
+            for fl, terms in leaves:
+                self.leaf_fid.append(self.dev(fl))
+                self.leaf_terms.append(terms)
+            prev = pre
+        self.tasks.append((r0, len(self.runs), out_slot))
+        self.task_terms.append(const_terms)
+        return len(self.tasks) - 1
+
+    def cost(self, t, leaf_w=1.0, run_w=2.0):
+        r0, r1, _ = self.tasks[t]
+        return leaf_w * sum(self.runs[r][5] for r in range(r0, r1)) + run_w * (r1 - r0)
+
+
+def split_poly(poly, nparts):
+    """Split a polynomial into ``nparts`` with balanced cost, keeping whole
+    prefix groups (runs) together."""
+    groups = defaultdict(list)
+    for mono in poly:
+        groups[mono[:-1]].append(mono)
+    keys = sorted(groups, key=lambda p: (-len(groups[p]), p))
+    parts = [dict() for _ in range(nparts)]
+    load = [0.0] * nparts
+    for p in keys:
+        i = int(np.argmin(load))
+        for mono in groups[p]:
+            parts[i][mono] = poly[mono]
+        load[i] += len(groups[p]) + 2.0
+    return [p for p in parts if p]
+
+
+def lpt_assign(costs, W):
+    """Longest-processing-time assignment of tasks to W waves."""
+    order = sorted(range(len(costs)), key=lambda t: -costs[t])
+    load = [0.0] * W
+    lists = [[] for _ in range(W)]
+    for t in order:
+        w = int(np.argmin(load))
+        lists[w].append(t)
+        load[w] += costs[t]
+    for l in lists:
+        l.sort()
+    return lists, load

@@ -1,0 +1,1255 @@
+// libacehip.so — MI355X (gfx950) basis-evaluation path of the Atomic Cluster
+// Expansion with ACE.jl v0.12.46 semantics.  FP64 throughout.
+//
+// Pipeline of an energy (+ forces) call:
+//   k_pool   : one wave per environment, lanes = neighbours.  Radial basis
+//              (orthpolys.jl:253-263 after distancetransforms.jl:73) and the
+//              m >= 0 complex spherical harmonics (sphericalharmonics.jl:
+//              43-51,167-197,379-403) per neighbour, staged in LDS, then
+//              lanes = pooled 1p functions sum over neighbours in order
+//              (product_1pbasis.jl:99-134) -> A+[k][env] in HBM.
+//   k_dag    : 64 environments per workgroup (lanes = environments), the
+//              A+ tile staged in LDS, every wave interprets its share of the
+//              run programs (depth-first product DAG: prefix products in
+//              registers, leaves streamed through scalar loads).  Computes
+//              the site energy sum c~ Re AA (evaluator.jl:121-147) and the
+//              adjoint polynomials G[k] (evaluator.jl:161-185 restated on A+).
+//   k_force  : one wave per environment, lanes = neighbours; recomputes the
+//              radial/Ylm values and gradients (orthpolys.jl:287-306,
+//              sphericalharmonics.jl:212-267,410-443,60-65) and contracts
+//              them with G (evaluator.jl:187-192): +dE/dr_j.
+// Basis-level kernels (A, AA, B = A2B AA, dB) follow pibasis.jl:265-275,
+// 338-432 and symmbasis.jl:248-336.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/acehip.h"
+#include "pack_host.hpp"
+
+#define ACE_LMAX 10
+#define ACE_NPMAX ((ACE_LMAX + 1) * (ACE_LMAX + 2) / 2)
+#define ACE_NRMAX 32
+#define ACE_CONJ 0x8000
+
+// ---------------------------------------------------------------------------
+// error handling
+
+static thread_local std::string g_err;
+
+static int set_err(const std::string& s) {
+    g_err = s;
+    return -1;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return set_err(std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// universal ALP recursion coefficients (sphericalharmonics.jl:146-159)
+
+__constant__ double c_alpA[ACE_NPMAX];
+__constant__ double c_alpB[ACE_NPMAX];
+
+__host__ __device__ constexpr int idxp(int l, int m) { return m + l * (l + 1) / 2; }
+
+struct RadialParams {
+    const double* A;
+    const double* B;
+    const double* C;
+    double r0, tl, tr;
+    int nr, tp, pl, pr;
+};
+
+__device__ __forceinline__ double ipow(double x, int p) {
+    double y = 1.0;
+    for (int i = 0; i < p; ++i) y *= x;
+    return y;
+}
+
+// t(r) and dt/dr for t = ((1 + r0)/(1 + r))^tp
+__device__ __forceinline__ void transform_ed(const RadialParams& rp, double r, double& t,
+                                             double& dt) {
+    double u = (1.0 + rp.r0) / (1.0 + r);
+    double du = -(1.0 + rp.r0) / ((1.0 + r) * (1.0 + r));
+    t = ipow(u, rp.tp);
+    dt = (double)rp.tp * ipow(u, rp.tp - 1) * du;
+}
+
+__device__ __forceinline__ double fcut(const RadialParams& rp, double t) {
+    if ((rp.pl > 0 && t < rp.tl) || (rp.pr > 0 && t > rp.tr)) return 0.0;
+    return ipow(t - rp.tl, rp.pl) * ipow(t - rp.tr, rp.pr);
+}
+
+__device__ __forceinline__ double fcut_d(const RadialParams& rp, double t) {
+    if ((rp.pl > 0 && t < rp.tl) || (rp.pr > 0 && t > rp.tr)) return 0.0;
+    double d = 0.0;
+    if (rp.pl > 0) d += rp.pl * ipow(t - rp.tl, rp.pl - 1) * ipow(t - rp.tr, rp.pr);
+    if (rp.pr > 0) d += ipow(t - rp.tl, rp.pl) * rp.pr * ipow(t - rp.tr, rp.pr - 1);
+    return d;
+}
+
+// Spherical coordinates; cos/sin of the azimuth from x/rho, y/rho.
+struct Sph {
+    double r, cphi, sphi, cth, sth;
+};
+
+__device__ __forceinline__ Sph cart2spher(double x, double y, double z) {
+    Sph s;
+    s.r = sqrt(x * x + y * y + z * z);
+    double rho = sqrt(x * x + y * y);
+    if (rho > 0.0) {
+        s.cphi = x / rho;
+        s.sphi = y / rho;
+    } else {
+        s.cphi = 1.0;
+        s.sphi = 0.0;
+    }
+    s.cth = z / s.r;
+    s.sth = rho / s.r;
+    return s;
+}
+
+// ALP values, sphericalharmonics.jl:167-197
+template <int L>
+__device__ __forceinline__ void alp_eval(const Sph& S, double* P) {
+    double temp = 0.3989422804014327;  // sqrt(1/(2 pi))
+    P[0] = temp;
+    if (L == 0) return;
+    P[idxp(1, 0)] = S.cth * 1.7320508075688772 * temp;
+    temp = -1.224744871391589 * S.sth * temp;
+    P[idxp(1, 1)] = temp;
+#pragma unroll
+    for (int l = 2; l <= L; ++l) {
+#pragma unroll
+        for (int m = 0; m <= l - 2; ++m)
+            P[idxp(l, m)] = c_alpA[idxp(l, m)] *
+                            (S.cth * P[idxp(l - 1, m)] + c_alpB[idxp(l, m)] * P[idxp(l - 2, m)]);
+        P[idxp(l, l - 1)] = S.cth * sqrt(2.0 * (l - 1) + 3.0) * temp;
+        temp = -sqrt(1.0 + 0.5 / l) * S.sth * temp;
+        P[idxp(l, l)] = temp;
+    }
+}
+
+// ALP values (P/sin(theta) for m >= 1) and d/dtheta, sphericalharmonics.jl:212-267
+template <int L>
+__device__ __forceinline__ void alp_eval_ed(const Sph& S, double* P, double* dP) {
+    double temp = 0.3989422804014327;
+    P[0] = temp;
+    dP[0] = 0.0;
+    if (L == 0) return;
+    const double s3 = 1.7320508075688772;
+    P[idxp(1, 0)] = S.cth * s3 * temp;
+    dP[idxp(1, 0)] = -S.sth * s3 * temp;
+    double temp1 = -1.224744871391589 * temp;
+    double temp_d = -1.224744871391589 * (S.cth * temp);
+    P[idxp(1, 1)] = temp1;
+    dP[idxp(1, 1)] = temp_d;
+    const double st2 = S.sth * S.sth;
+#pragma unroll
+    for (int l = 2; l <= L; ++l) {
+        {
+            const int i = idxp(l, 0), i1 = idxp(l - 1, 0), i2 = idxp(l - 2, 0);
+            P[i] = c_alpA[i] * (S.cth * P[i1] + c_alpB[i] * P[i2]);
+            dP[i] = c_alpA[i] * (-S.sth * P[i1] + S.cth * dP[i1] + c_alpB[i] * dP[i2]);
+        }
+#pragma unroll
+        for (int m = 1; m <= l - 2; ++m) {
+            const int i = idxp(l, m), i1 = idxp(l - 1, m), i2 = idxp(l - 2, m);
+            P[i] = c_alpA[i] * (S.cth * P[i1] + c_alpB[i] * P[i2]);
+            dP[i] = c_alpA[i] * (-st2 * P[i1] + S.cth * dP[i1] + c_alpB[i] * dP[i2]);
+        }
+        const double c = sqrt(2.0 * (l - 1) + 3.0);
+        P[idxp(l, l - 1)] = c * S.cth * temp1;
+        dP[idxp(l, l - 1)] = c * (-st2 * temp1 + S.cth * temp_d);
+        const double f = sqrt(1.0 + 0.5 / l);
+        const double nt1 = -f * S.sth * temp1;
+        const double ntd = -f * (S.cth * temp1 * S.sth + S.sth * temp_d);
+        temp1 = nt1;
+        temp_d = ntd;
+        P[idxp(l, l)] = temp1;
+        dP[idxp(l, l)] = temp_d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_pool: A+[k][env]
+
+template <int L>
+__global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
+                                              const int32_t* __restrict__ off,
+                                              const double* __restrict__ xyz, int64_t nnbr,
+                                              const int8_t* __restrict__ species,
+                                              const int32_t* __restrict__ pool, int npool,
+                                              double2* __restrict__ Aplus, int64_t estride) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    const int64_t env = blockIdx.x;
+    const int nr = rp.nr;
+    const int rstride = nr + 2 * NP + 1;  // doubles per neighbour slot
+    double* base = smem;
+    if (env >= nenv) return;
+    const int j0 = off[env], j1 = off[env + 1];
+
+    for (int jb = j0; jb < j1 || jb == j0; jb += 64) {
+        const int nch = max(0, min(64, j1 - jb));
+        if (lane < nch) {
+            const int j = jb + lane;
+            const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+            double* slot = base + lane * rstride;
+            Sph S = cart2spher(x, y, z);
+            // radial basis
+            double u = (1.0 + rp.r0) / (1.0 + S.r);
+            double t = ipow(u, rp.tp);
+            double p1 = rp.A[0] * fcut(rp, t);
+            slot[0] = p1;
+            double p0 = 0.0;
+            if (nr > 1) {
+                p0 = p1;
+                p1 = (rp.A[1] * t + rp.B[1]) * p0;
+                slot[1] = p1;
+            }
+            for (int n = 2; n < nr; ++n) {
+                double p2 = (rp.A[n] * t + rp.B[n]) * p1 + rp.C[n] * p0;
+                p0 = p1;
+                p1 = p2;
+                slot[n] = p2;
+            }
+            double P[NP];
+            alp_eval<L>(S, P);
+            double* ys = slot + nr;
+            const double isq2 = 0.7071067811865475;
+#pragma unroll
+            for (int l = 0; l <= L; ++l) {
+                ys[2 * idxp(l, 0)] = P[idxp(l, 0)] * isq2;
+                ys[2 * idxp(l, 0) + 1] = 0.0;
+            }
+            double epr = isq2, epi = 0.0;
+#pragma unroll
+            for (int m = 1; m <= L; ++m) {
+                const double nr_ = epr * S.cphi - epi * S.sphi;
+                const double ni_ = epr * S.sphi + epi * S.cphi;
+                epr = nr_;
+                epi = ni_;
+#pragma unroll
+                for (int l = m; l <= L; ++l) {
+                    const double p = P[idxp(l, m)];
+                    ys[2 * idxp(l, m)] = epr * p;
+                    ys[2 * idxp(l, m) + 1] = epi * p;
+                }
+            }
+            slot[nr + 2 * NP] = species ? (double)species[j] : -1.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // lanes over pooled functions; partial sums of earlier neighbour
+        // chunks are read back from HBM (same lane wrote them)
+        for (int k = lane; k < npool; k += 64) {
+            const int mu = pool[4 * k], n = pool[4 * k + 1] - 1, l = pool[4 * k + 2],
+                      m = pool[4 * k + 3];
+            const int iy = idxp(l, m);
+            double ar = 0.0, ai = 0.0;
+            if (jb != j0) {
+                const double2 prev = Aplus[(int64_t)k * estride + env];
+                ar = prev.x;
+                ai = prev.y;
+            }
+            for (int jj = 0; jj < nch; ++jj) {
+                const double* slot = base + jj * rstride;
+                if (mu >= 0 && (int)slot[nr + 2 * NP] != mu) continue;
+                const double R = slot[n];
+                ar += R * slot[nr + 2 * iy];
+                ai += R * slot[nr + 2 * iy + 1];
+            }
+            Aplus[(int64_t)k * estride + env] = make_double2(ar, ai);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_dag: run-program interpreter, lanes = environments
+
+struct DagArgs {
+    const int32_t* runs;       // [nruns][7]
+    const int32_t* leaf_fid;   // [nleaves]
+    const double* leaf_coef;   // [nleaves][ncoef]
+    const int32_t* tasks;      // [ntasks][3]
+    const double* task_const;  // [ntasks][2 | 6]
+    const int32_t* wtask_ptr;  // [W+1]
+    const int32_t* wtask_list;
+    int npool;
+    int W;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double2 lds_factor(const double2* As, int f, int lane) {
+    double2 v = As[(f & 0x7fff) * 64 + lane];
+    if (f & ACE_CONJ) v.y = -v.y;
+    return v;
+}
+
+// PROP 0: real coefficient, output Re(acc) (energy) or acc (G slot).
+// PROP 1: 3 complex coefficients, output Re(acc_p) or acc_p.
+template <int PROP>
+__global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
+                                              const double2* __restrict__ Aplus,
+                                              int64_t estride, double* __restrict__ E_out,
+                                              double2* __restrict__ Gbuf, double e0r,
+                                              const double* __restrict__ e0v) {
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    extern __shared__ double2 As[];  // [npool][64]
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t env = (int64_t)blockIdx.x * 64 + lane;
+    const bool valid = env < nenv;
+    for (int k = wave; k < a.npool; k += a.W)
+        As[k * 64 + lane] = valid ? Aplus[(int64_t)k * estride + env] : make_double2(0.0, 0.0);
+    __syncthreads();
+
+    double ep[NC];
+#pragma unroll
+    for (int p = 0; p < NC; ++p) ep[p] = 0.0;
+
+    const int tb = a.wtask_ptr[wave], te = a.wtask_ptr[wave + 1];
+    for (int ti = tb; ti < te; ++ti) {
+        const int t = a.wtask_list[ti];
+        const int rb = a.tasks[3 * t], re = a.tasks[3 * t + 1], slot = a.tasks[3 * t + 2];
+        double2 acc[NC];
+#pragma unroll
+        for (int p = 0; p < NC; ++p)
+            acc[p] = make_double2(a.task_const[(2 * NC) * t + 2 * p],
+                                  a.task_const[(2 * NC) * t + 2 * p + 1]);
+        double2 P1 = make_double2(1.0, 0.0), P2 = P1, P3 = P1;
+        for (int r = rb; r < re; ++r) {
+            const int* run = a.runs + 7 * r;
+            const int d = run[0], j = run[1];
+            const int nl = run[5], lo = run[6];
+            if (j <= 1 && d >= 1) P1 = lds_factor(As, run[2], lane);
+            if (j <= 2 && d >= 2) P2 = cmul(P1, lds_factor(As, run[3], lane));
+            if (j <= 3 && d >= 3) P3 = cmul(P2, lds_factor(As, run[4], lane));
+            const double2 Pd = d == 0 ? make_double2(1.0, 0.0) : d == 1 ? P1 : d == 2 ? P2 : P3;
+            double2 S[NC];
+#pragma unroll
+            for (int p = 0; p < NC; ++p) S[p] = make_double2(0.0, 0.0);
+            for (int i = 0; i < nl; ++i) {
+                const int f = a.leaf_fid[lo + i];
+                const double2 v = lds_factor(As, f, lane);
+                if (PROP == 0) {
+                    const double c = a.leaf_coef[lo + i];
+                    S[0].x += c * v.x;
+                    S[0].y += c * v.y;
+                } else {
+                    const double* c = a.leaf_coef + 6 * (int64_t)(lo + i);
+#pragma unroll
+                    for (int p = 0; p < NC; ++p) {
+                        S[p].x += c[2 * p] * v.x - c[2 * p + 1] * v.y;
+                        S[p].y += c[2 * p] * v.y + c[2 * p + 1] * v.x;
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < NC; ++p) {
+                acc[p].x += Pd.x * S[p].x - Pd.y * S[p].y;
+                acc[p].y += Pd.x * S[p].y + Pd.y * S[p].x;
+            }
+        }
+        if (slot < 0) {
+#pragma unroll
+            for (int p = 0; p < NC; ++p) ep[p] += acc[p].x;
+        } else if (valid) {
+#pragma unroll
+            for (int p = 0; p < NC; ++p) Gbuf[((int64_t)slot * NC + p) * estride + env] = acc[p];
+        }
+    }
+    // reduce energy partials over waves (reuse the A tile region)
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(As);
+#pragma unroll
+    for (int p = 0; p < NC; ++p) red[(wave * NC + p) * 64 + lane] = ep[p];
+    __syncthreads();
+    if (wave == 0 && valid && E_out) {
+#pragma unroll
+        for (int p = 0; p < NC; ++p) {
+            double s = PROP == 0 ? e0r : e0v[p];
+            for (int w = 0; w < a.W; ++w) s += red[(w * NC + p) * 64 + lane];
+            E_out[env * NC + p] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_force: +dE/dr_j = Re sum_k G[k] grad(R_n Y_lm)(r_j)
+
+template <int L, int PROP>
+__global__ __launch_bounds__(64) void k_force(RadialParams rp, int64_t nenv,
+                                               const int32_t* __restrict__ off,
+                                               const double* __restrict__ xyz, int64_t nnbr,
+                                               const int8_t* __restrict__ species,
+                                               const int16_t* __restrict__ kidx, int npool,
+                                               const int32_t* __restrict__ gslot_ptr,
+                                               const int32_t* __restrict__ gslot_list,
+                                               const double2* __restrict__ Gbuf, int64_t estride,
+                                               double* __restrict__ dE) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    const int64_t env = blockIdx.x;
+    const int nr = rp.nr;
+    double2* Gs = reinterpret_cast<double2*>(smem);
+    double* Rs = reinterpret_cast<double*>(Gs + npool * NC);  // [64][2*nr]
+    if (env >= nenv) return;
+    for (int q = lane; q < npool * NC; q += 64) {
+        const int k = q / NC, p = q % NC;
+        double2 g = make_double2(0.0, 0.0);
+        for (int s = gslot_ptr[k]; s < gslot_ptr[k + 1]; ++s) {
+            const double2 v = Gbuf[((int64_t)gslot_list[s] * NC + p) * estride + env];
+            g.x += v.x;
+            g.y += v.y;
+        }
+        Gs[q] = g;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int j0 = off[env], j1 = off[env + 1];
+    double* myR = Rs + lane * 2 * nr;
+    for (int j = j0 + lane; j < j1; j += 64) {
+        const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+        const int mu = species ? (int)species[j] : 0;
+        Sph S = cart2spher(x, y, z);
+        double t, dtdr;
+        transform_ed(rp, S.r, t, dtdr);
+        {
+            double p0 = rp.A[0] * fcut(rp, t), d0 = rp.A[0] * fcut_d(rp, t);
+            myR[0] = p0;
+            myR[nr] = d0;
+            if (nr > 1) {
+                double al = rp.A[1] * t + rp.B[1];
+                double p1 = al * p0, d1 = al * d0 + rp.A[1] * p0;
+                myR[1] = p1;
+                myR[nr + 1] = d1;
+                for (int n = 2; n < nr; ++n) {
+                    al = rp.A[n] * t + rp.B[n];
+                    const double p2 = al * p1 + rp.C[n] * p0;
+                    const double d2 = al * d1 + rp.C[n] * d0 + rp.A[n] * p1;
+                    p0 = p1;
+                    p1 = p2;
+                    d0 = d1;
+                    d1 = d2;
+                    myR[n] = p2;
+                    myR[nr + n] = d2;
+                }
+            }
+        }
+        double P[NP], dP[NP];
+        alp_eval_ed<L>(S, P, dP);
+        const double rinv = 1.0 / (S.r + 2.220446049250313e-16);
+        const double rx = x / S.r, ry = y / S.r, rz = z / S.r;
+        double g[NC][3];
+#pragma unroll
+        for (int p = 0; p < NC; ++p) g[p][0] = g[p][1] = g[p][2] = 0.0;
+        double epr = 0.7071067811865475, epi = 0.0;
+#pragma unroll
+        for (int m = 0; m <= L; ++m) {
+            if (m > 0) {
+                const double nr_ = epr * S.cphi - epi * S.sphi;
+                const double ni_ = epr * S.sphi + epi * S.cphi;
+                epr = nr_;
+                epi = ni_;
+            }
+#pragma unroll
+            for (int l = m; l <= L; ++l) {
+                const int ip = idxp(l, m);
+                // Y and its Cartesian gradient (cYlm_ed!, dspher_to_dcart)
+                double Yr, Yi, fpr, fpi, ftr, fti;
+                if (m == 0) {
+                    Yr = P[ip] * 0.7071067811865475;
+                    Yi = 0.0;
+                    fpr = fpi = 0.0;
+                    ftr = dP[ip] * 0.7071067811865475;
+                    fti = 0.0;
+                } else {
+                    const double pds = P[ip];
+                    Yr = epr * pds * S.sth;
+                    Yi = epi * pds * S.sth;
+                    // dep/dphi = i m ep
+                    fpr = -m * epi * pds;
+                    fpi = m * epr * pds;
+                    ftr = epr * dP[ip];
+                    fti = epi * dP[ip];
+                }
+                const double dYr[3] = {(-S.sphi * fpr + S.cphi * S.cth * ftr) * rinv,
+                                       (S.cphi * fpr + S.sphi * S.cth * ftr) * rinv,
+                                       (-S.sth * ftr) * rinv};
+                const double dYi[3] = {(-S.sphi * fpi + S.cphi * S.cth * fti) * rinv,
+                                       (S.cphi * fpi + S.sphi * S.cth * fti) * rinv,
+                                       (-S.sth * fti) * rinv};
+                const int16_t* kr = kidx + ((int64_t)(mu * (L + 1) + l) * (L + 1) + m) * nr;
+                for (int n = 0; n < nr; ++n) {
+                    const int k = kr[n];
+                    if (k < 0) continue;
+                    const double R = myR[n], dR = myR[nr + n] * dtdr;
+#pragma unroll
+                    for (int p = 0; p < NC; ++p) {
+                        const double2 G = Gs[k * NC + p];
+                        // Re(G * (dR rhat Y + R dY))
+                        const double gy = G.x * Yr - G.y * Yi;
+                        g[p][0] += dR * gy * rx + R * (G.x * dYr[0] - G.y * dYi[0]);
+                        g[p][1] += dR * gy * ry + R * (G.x * dYr[1] - G.y * dYi[1]);
+                        g[p][2] += dR * gy * rz + R * (G.x * dYr[2] - G.y * dYi[2]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < NC; ++p)
+#pragma unroll
+            for (int dd = 0; dd < 3; ++dd) dE[(int64_t)(p * 3 + dd) * nnbr + j] = g[p][dd];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// basis level: A (full), AA, B, dB
+
+__global__ void k_expandA(int64_t nenv, int nA, const int32_t* __restrict__ fid,
+                          const double* __restrict__ sgn, const double2* __restrict__ Aplus,
+                          int64_t estride, double2* __restrict__ A_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nenv * nA) return;
+    const int64_t env = i / nA;
+    const int iA = (int)(i % nA);
+    const int f = fid[iA];
+    double2 v = Aplus[(int64_t)(f & 0x7fff) * estride + env];
+    if (f & ACE_CONJ) v.y = -v.y;
+    v.x *= sgn[iA];
+    v.y *= sgn[iA];
+    A_out[i] = v;
+}
+
+// AA[env][iAA] = prod_t A[iAA2iA[iAA][t]], left to right (pibasis.jl:265-275)
+__global__ void k_AA(int64_t nenv, int nA, int nAA, int maxord, const int32_t* __restrict__ orders,
+                     const int32_t* __restrict__ iAA2iA, const double2* __restrict__ A,
+                     double2* __restrict__ AA) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nenv * nAA) return;
+    const int64_t env = i / nAA;
+    const int r = (int)(i % nAA);
+    const double2* Ae = A + env * nA;
+    double2 aa = make_double2(1.0, 0.0);
+    const int o = orders[r];
+    for (int t = 0; t < o; ++t) aa = cmul(aa, Ae[iAA2iA[r * maxord + t]]);
+    AA[i] = aa;
+}
+
+// B[env][b][p] = Re sum_{nz in row b} val * AA
+template <int PROP>
+__global__ void k_B(int64_t nenv, int nB, int nAA, const int32_t* __restrict__ rowptr,
+                    const int32_t* __restrict__ cols, const double* __restrict__ vals,
+                    const double2* __restrict__ AA, double* __restrict__ B) {
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nenv * nB) return;
+    const int64_t env = i / nB;
+    const int b = (int)(i % nB);
+    double s[NC];
+#pragma unroll
+    for (int p = 0; p < NC; ++p) s[p] = 0.0;
+    for (int z = rowptr[b]; z < rowptr[b + 1]; ++z) {
+        const double2 a = AA[env * nAA + cols[z]];
+        if (PROP == 0) {
+            s[0] += vals[z] * a.x;  // isrealAA: AA = real(prod A)
+        } else {
+#pragma unroll
+            for (int p = 0; p < NC; ++p) s[p] += vals[6 * z + 2 * p] * a.x - vals[6 * z + 2 * p + 1] * a.y;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < NC; ++p) B[i * NC + p] = s[p];
+}
+
+// per-neighbour 1p gradients dphi[j][iA][3] (complex), product_1pbasis.jl:169-221
+template <int L>
+__global__ __launch_bounds__(64) void k_dphi(RadialParams rp, int64_t nnbr,
+                                             const int32_t* __restrict__ nbr_env,
+                                             const double* __restrict__ xyz,
+                                             const int8_t* __restrict__ species,
+                                             const int32_t* __restrict__ aspec4, int nA,
+                                             double2* __restrict__ dphi) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nnbr) return;
+    const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+    const int mu = species ? (int)species[j] : -1;
+    Sph S = cart2spher(x, y, z);
+    double t, dtdr;
+    transform_ed(rp, S.r, t, dtdr);
+    double R[ACE_NRMAX], dR[ACE_NRMAX];
+    R[0] = rp.A[0] * fcut(rp, t);
+    dR[0] = rp.A[0] * fcut_d(rp, t);
+    for (int n = 1; n < rp.nr && n < ACE_NRMAX; ++n) {
+        const double al = rp.A[n] * t + rp.B[n];
+        if (n == 1) {
+            R[1] = al * R[0];
+            dR[1] = al * dR[0] + rp.A[1] * R[0];
+        } else {
+            R[n] = al * R[n - 1] + rp.C[n] * R[n - 2];
+            dR[n] = al * dR[n - 1] + rp.C[n] * dR[n - 2] + rp.A[n] * R[n - 1];
+        }
+    }
+    double P[NP], dP[NP];
+    alp_eval_ed<L>(S, P, dP);
+    const double rinv = 1.0 / (S.r + 2.220446049250313e-16);
+    const double rh[3] = {x / S.r, y / S.r, z / S.r};
+    for (int iA = 0; iA < nA; ++iA) {
+        const int amu = aspec4[4 * iA], n = aspec4[4 * iA + 1] - 1, l = aspec4[4 * iA + 2],
+                  mm = aspec4[4 * iA + 3];
+        double2* out = dphi + (j * nA + iA) * 3;
+        if (amu >= 0 && amu != mu) {
+            for (int d = 0; d < 3; ++d) out[d] = make_double2(0.0, 0.0);
+            continue;
+        }
+        const int m = mm < 0 ? -mm : mm;
+        // e^{i m phi}/sqrt2
+        double epr = 0.7071067811865475, epi = 0.0;
+        for (int q = 0; q < m; ++q) {
+            const double a = epr * S.cphi - epi * S.sphi;
+            const double b = epr * S.sphi + epi * S.cphi;
+            epr = a;
+            epi = b;
+        }
+        double pds = 0.0, dpt = 0.0;
+        for (int ll = 0; ll <= L; ++ll)
+            for (int q = 0; q <= ll; ++q)
+                if (ll == l && q == m) {
+                    pds = P[idxp(ll, q)];
+                    dpt = dP[idxp(ll, q)];
+                }
+        double Yr, Yi, fpr, fpi, ftr, fti;
+        if (m == 0) {
+            Yr = pds * 0.7071067811865475;
+            Yi = 0.0;
+            fpr = fpi = 0.0;
+            ftr = dpt * 0.7071067811865475;
+            fti = 0.0;
+        } else {
+            Yr = epr * pds * S.sth;
+            Yi = epi * pds * S.sth;
+            fpr = -m * epi * pds;
+            fpi = m * epr * pds;
+            ftr = epr * dpt;
+            fti = epi * dpt;
+        }
+        if (mm < 0) {  // Y_l^-m = (-1)^m conj(Y_l^m), same for the gradient
+            const double s = (m & 1) ? -1.0 : 1.0;
+            Yr *= s;
+            Yi *= -s;
+            fpr *= s;
+            fpi *= -s;
+            ftr *= s;
+            fti *= -s;
+        }
+        const double dYr[3] = {(-S.sphi * fpr + S.cphi * S.cth * ftr) * rinv,
+                               (S.cphi * fpr + S.sphi * S.cth * ftr) * rinv,
+                               (-S.sth * ftr) * rinv};
+        const double dYi[3] = {(-S.sphi * fpi + S.cphi * S.cth * fti) * rinv,
+                               (S.cphi * fpi + S.sphi * S.cth * fti) * rinv,
+                               (-S.sth * fti) * rinv};
+        const double Rv = R[n], dRv = dR[n] * dtdr;
+        for (int d = 0; d < 3; ++d)
+            out[d] = make_double2(dRv * rh[d] * Yr + Rv * dYr[d], dRv * rh[d] * Yi + Rv * dYi[d]);
+    }
+}
+
+// dAA[j][iAA][3] = sum_a (prod_{t != a} A_t) dphi[j][v_a]  (pibasis.jl:402-432)
+__global__ void k_dAA(int64_t nnbr, int nA, int nAA, int maxord, const int32_t* __restrict__ nbr_env,
+                      const int32_t* __restrict__ orders, const int32_t* __restrict__ iAA2iA,
+                      const double2* __restrict__ A, const double2* __restrict__ dphi,
+                      double2* __restrict__ dAA) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnbr * nAA) return;
+    const int64_t j = i / nAA;
+    const int r = (int)(i % nAA);
+    const int64_t env = nbr_env[j];
+    const double2* Ae = A + env * nA;
+    const int o = orders[r];
+    double2 g[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    for (int a = 0; a < o; ++a) {
+        double2 w = make_double2(1.0, 0.0);
+        for (int t = 0; t < o; ++t)
+            if (t != a) w = cmul(w, Ae[iAA2iA[r * maxord + t]]);
+        const double2* dp = dphi + (j * nA + iAA2iA[r * maxord + a]) * 3;
+        for (int d = 0; d < 3; ++d) {
+            const double2 v = cmul(w, dp[d]);
+            g[d].x += v.x;
+            g[d].y += v.y;
+        }
+    }
+    for (int d = 0; d < 3; ++d) dAA[(i * 3) + d] = g[d];
+}
+
+template <int PROP>
+__global__ void k_dB(int64_t nnbr, int nB, int nAA, const int32_t* __restrict__ rowptr,
+                     const int32_t* __restrict__ cols, const double* __restrict__ vals,
+                     const double2* __restrict__ dAA, double* __restrict__ dB) {
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnbr * nB) return;
+    const int64_t j = i / nB;
+    const int b = (int)(i % nB);
+    double s[NC][3];
+#pragma unroll
+    for (int p = 0; p < NC; ++p) s[p][0] = s[p][1] = s[p][2] = 0.0;
+    for (int z = rowptr[b]; z < rowptr[b + 1]; ++z) {
+        const double2* g = dAA + ((j * nAA) + cols[z]) * 3;
+        for (int d = 0; d < 3; ++d) {
+            if (PROP == 0) {
+                s[0][d] += vals[z] * g[d].x;
+            } else {
+#pragma unroll
+                for (int p = 0; p < NC; ++p)
+                    s[p][d] += vals[6 * z + 2 * p] * g[d].x - vals[6 * z + 2 * p + 1] * g[d].y;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < NC; ++p)
+        for (int d = 0; d < 3; ++d) dB[(i * NC + p) * 3 + d] = s[p][d];
+}
+
+__global__ void k_nbr_env(int64_t nenv, const int32_t* __restrict__ off, int32_t* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nenv) return;
+    for (int j = off[e]; j < off[e + 1]; ++j) out[j] = (int32_t)e;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int ensure(size_t count) {
+        if (count <= n) return 0;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+        }
+        HIP_TRY(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        n = count;
+        return 0;
+    }
+    int upload(const T* h, size_t count) {
+        if (ensure(count)) return -1;
+        if (count) HIP_TRY(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    }
+};
+
+struct WaveCfg {
+    int W;
+    std::vector<int32_t> ptr_all, list_all, ptr_e, list_e;
+    DevBuf<int32_t> d_ptr_all, d_list_all, d_ptr_e, d_list_e;
+};
+
+struct ace_ctx {
+    int device = 0;
+    int prop = 0, L = 0, nr = 0, nspecies = 0, tp = 2, pl = 0, pr = 0;
+    double r0 = 1.0, tl = 0, tr = 0;
+    int npool = 0, nA = 0, nAA = 0, maxorder = 0, nB = 0, nnz = 0;
+    int nruns = 0, nleaves = 0, ntasks = 0, nterms = 0, ngslots = 0;
+    int ncoef = 1;
+    acehost::PackCopy host;
+    // device
+    DevBuf<double> rA, rB, rC;
+    DevBuf<int32_t> pool, aspec_fid, aspec4, orders, iAA2iA, rowptr, cols;
+    DevBuf<double> aspec_sgn, vals;
+    DevBuf<int16_t> kidx;
+    DevBuf<int32_t> runs, leaf_fid, d_tasks, gslot_ptr, gslot_list;
+    DevBuf<double> leaf_coef, task_const, e0v;
+    double e0r = 0.0;
+    std::vector<WaveCfg*> wcfg;
+    int cur_w = 0;
+    // scratch
+    DevBuf<double2> Aplus, Gbuf, Afull, AA, dphi, dAA;
+    DevBuf<int32_t> nbr_env;
+    // timing
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timing_valid = false;
+    bool validate = true;
+    ~ace_ctx() {
+        for (auto* w : wcfg) delete w;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+static int upload_coeffs(ace_ctx* ctx, const double* c) {
+    acehost::Coeffs cf = acehost::coefficients(ctx->host, c);
+    if (ctx->leaf_coef.upload(cf.leaf.data(), cf.leaf.size())) return -1;
+    if (ctx->task_const.upload(cf.task.data(), cf.task.size())) return -1;
+    ctx->e0r = cf.e0[0];
+    double ev[3] = {cf.e0[0], cf.e0[2], cf.e0[4]};
+    return ctx->e0v.upload(ev, 3);
+}
+
+extern "C" int ace_abi_version(void) { return ACEHIP_ABI_VERSION; }
+
+extern "C" const char* ace_last_error(void) { return g_err.c_str(); }
+
+static bool g_alp_ready[64] = {false};
+
+static int ensure_alp(int device) {
+    if (device < 0 || device >= 64) return set_err("device index out of range");
+    if (g_alp_ready[device]) return 0;
+    std::vector<double> A(ACE_NPMAX, 0.0), B(ACE_NPMAX, 0.0);
+    for (int l = 2; l <= ACE_LMAX; ++l) {
+        const double ls = (double)l * l, lm1s = (double)(l - 1) * (l - 1);
+        for (int m = 0; m <= l - 2; ++m) {
+            const double ms = (double)m * m;
+            A[idxp(l, m)] = std::sqrt((4 * ls - 1.0) / (ls - ms));
+            B[idxp(l, m)] = -std::sqrt((lm1s - ms) / (4 * lm1s - 1.0));
+        }
+    }
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_alpA), A.data(), sizeof(double) * ACE_NPMAX));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_alpB), B.data(), sizeof(double) * ACE_NPMAX));
+    g_alp_ready[device] = true;
+    return 0;
+}
+
+static int set_func_lds(const void* f, size_t bytes) {
+    if (bytes > 64 * 1024) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return 0;
+}
+
+extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, int64_t nB,
+                              ace_ctx** out) {
+    if (!pk || !out || !c) return set_err("ace_ctx_create: null argument");
+    *out = nullptr;
+    const std::string verr = acehost::validate_pack(pk);
+    if (!verr.empty()) return set_err("ace_ctx_create: " + verr);
+    if (nB != pk->nB) return set_err("ace_ctx_create: len(c) != len(basis)");
+    std::vector<acehost::WaveTable> wt;
+    const std::string werr = acehost::wave_tables(pk, wt);
+    if (!werr.empty()) return set_err("ace_ctx_create: " + werr);
+    HIP_TRY(hipSetDevice(device));
+    if (ensure_alp(device)) return -1;
+    ace_ctx* ctx = new ace_ctx();
+    ctx->device = device;
+    ctx->prop = pk->prop;
+    ctx->L = pk->L;
+    ctx->nr = pk->nr;
+    ctx->nspecies = pk->nspecies;
+    ctx->tp = pk->tp;
+    ctx->pl = pk->pl;
+    ctx->pr = pk->pr;
+    ctx->r0 = pk->r0;
+    ctx->tl = pk->tl;
+    ctx->tr = pk->tr;
+    ctx->npool = pk->npool;
+    ctx->nA = pk->nA;
+    ctx->nAA = pk->nAA;
+    ctx->maxorder = pk->maxorder;
+    ctx->nB = pk->nB;
+    ctx->nnz = pk->a2b_nnz;
+    ctx->nruns = pk->nruns;
+    ctx->nleaves = pk->nleaves;
+    ctx->ntasks = pk->ntasks;
+    ctx->nterms = pk->nterms;
+    ctx->ngslots = pk->ngslots;
+    ctx->ncoef = pk->prop == ACE_PROP_INVARIANT ? 1 : 6;
+    ctx->host = acehost::copy_pack(pk);
+    const int nc = ctx->ncoef;
+    auto fail = [&](const char* what) {
+        std::string msg = g_err;
+        delete ctx;
+        return set_err(std::string("ace_ctx_create: ") + what + ": " + msg);
+    };
+    if (ctx->rA.upload(pk->rA, pk->nr) || ctx->rB.upload(pk->rB, pk->nr) ||
+        ctx->rC.upload(pk->rC, pk->nr) || ctx->pool.upload(pk->pool, 4 * (size_t)pk->npool) ||
+        ctx->aspec_fid.upload(pk->aspec_fid, pk->nA) ||
+        ctx->aspec_sgn.upload(pk->aspec_sgn, pk->nA) ||
+        ctx->orders.upload(pk->orders, pk->nAA) ||
+        ctx->iAA2iA.upload(pk->iAA2iA, (size_t)pk->nAA * pk->maxorder) ||
+        ctx->rowptr.upload(pk->a2b_rowptr, pk->nB + 1) ||
+        ctx->cols.upload(pk->a2b_cols, pk->a2b_nnz) ||
+        ctx->vals.upload(pk->a2b_vals, (size_t)pk->a2b_nnz * nc) ||
+        ctx->runs.upload(pk->runs, 7 * (size_t)pk->nruns) ||
+        ctx->leaf_fid.upload(pk->leaf_fid, pk->nleaves) ||
+        ctx->d_tasks.upload(pk->tasks, 3 * (size_t)pk->ntasks) ||
+        ctx->gslot_ptr.upload(pk->gslot_ptr, pk->npool + 1) ||
+        ctx->gslot_list.upload(pk->gslot_list, pk->gslot_ptr[pk->npool]))
+        return fail("upload");
+    {
+        std::vector<int32_t> a4 = acehost::aspec_rows(pk);
+        if (ctx->aspec4.upload(a4.data(), a4.size())) return fail("upload");
+        std::vector<int16_t> kx = acehost::pool_lookup(pk);
+        if (ctx->kidx.upload(kx.data(), kx.size())) return fail("upload");
+    }
+    for (auto& t : wt) {
+        WaveCfg* wc = new WaveCfg();
+        ctx->wcfg.push_back(wc);
+        wc->W = t.W;
+        wc->ptr_all = t.ptr_all;
+        wc->list_all = t.list_all;
+        wc->ptr_e = t.ptr_e;
+        wc->list_e = t.list_e;
+        if (wc->d_ptr_all.upload(wc->ptr_all.data(), wc->ptr_all.size()) ||
+            wc->d_list_all.upload(wc->list_all.data(), wc->list_all.size()) ||
+            wc->d_ptr_e.upload(wc->ptr_e.data(), wc->ptr_e.size()) ||
+            wc->d_list_e.upload(wc->list_e.data(), wc->list_e.size()))
+            return fail("upload");
+    }
+    if (upload_coeffs(ctx, c)) return fail("set_params");
+    for (auto& e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail("event");
+    *out = ctx;
+    return 0;
+}
+
+extern "C" int ace_set_params(ace_ctx* ctx, const double* c, int64_t nB) {
+    if (!ctx || !c) return set_err("ace_set_params: null argument");
+    if (nB != ctx->nB) return set_err("ace_set_params: len(c) != len(basis)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return upload_coeffs(ctx, c);
+}
+
+extern "C" int ace_set_tile(ace_ctx* ctx, int W) {
+    if (!ctx) return set_err("ace_set_tile: null context");
+    for (size_t q = 0; q < ctx->wcfg.size(); ++q)
+        if (ctx->wcfg[q]->W == W) {
+            ctx->cur_w = (int)q;
+            return 0;
+        }
+    return set_err("ace_set_tile: waves per tile not in the pack");
+}
+
+extern "C" int ace_set_validation(ace_ctx* ctx, int on) {
+    if (!ctx) return set_err("ace_set_validation: null context");
+    ctx->validate = on != 0;
+    return 0;
+}
+
+extern "C" void ace_ctx_destroy(ace_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    delete ctx;
+}
+
+static RadialParams radial_params(ace_ctx* ctx) {
+    RadialParams rp;
+    rp.A = ctx->rA.p;
+    rp.B = ctx->rB.p;
+    rp.C = ctx->rC.p;
+    rp.r0 = ctx->r0;
+    rp.tl = ctx->tl;
+    rp.tr = ctx->tr;
+    rp.nr = ctx->nr;
+    rp.tp = ctx->tp;
+    rp.pl = ctx->pl;
+    rp.pr = ctx->pr;
+    return rp;
+}
+
+static int check_common(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                        const int8_t* species, int64_t* nnbr_out) {
+    if (!ctx) return set_err("null context");
+    if (nenv < 0) return set_err("nenv < 0");
+    if (nenv > 0 && (!off || !xyz)) return set_err("null env_offsets / xyz");
+    if (ctx->nspecies > 0 && nenv > 0 && !species)
+        return set_err("species array required for a categorical 1p basis");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int32_t last = 0;
+    if (nenv > 0) {
+        int32_t first = 0;
+        HIP_TRY(hipMemcpy(&first, off, sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&last, off + nenv, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (first != 0 || last < 0) return set_err("env_offsets must start at 0 and be non-negative");
+    }
+    *nnbr_out = last;
+    return 0;
+}
+
+// host-side validation of offsets and species (bounds for every kernel)
+static int validate_inputs(ace_ctx* ctx, int64_t nenv, const int32_t* off, int64_t nnbr,
+                           const int8_t* species) {
+    if (nenv == 0 || !ctx->validate) return 0;
+    std::vector<int32_t> h(nenv + 1);
+    HIP_TRY(hipMemcpy(h.data(), off, sizeof(int32_t) * (nenv + 1), hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < nenv; ++e)
+        if (h[e + 1] < h[e]) return set_err("env_offsets must be non-decreasing");
+    if (species && ctx->nspecies > 0 && nnbr > 0) {
+        std::vector<int8_t> s(nnbr);
+        HIP_TRY(hipMemcpy(s.data(), species, nnbr, hipMemcpyDeviceToHost));
+        for (int64_t j = 0; j < nnbr; ++j)
+            if (s[j] < 0 || s[j] >= ctx->nspecies) return set_err("species index out of range (unknown category)");
+    }
+    return 0;
+}
+
+template <int L>
+static void launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
+                        const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    const size_t lds = (size_t)64 * (ctx->nr + 2 * NP + 1) * sizeof(double);
+    set_func_lds((const void*)k_pool<L>, lds);
+    hipLaunchKernelGGL(k_pool<L>, dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx), nenv,
+                       off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->pool.p,
+                       ctx->npool, ctx->Aplus.p, estride);
+}
+
+template <int L, int PROP>
+static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
+                         const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride,
+                         double* dE) {
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    const size_t lds = (size_t)ctx->npool * NC * sizeof(double2) + 64 * 2 * ctx->nr * sizeof(double);
+    set_func_lds((const void*)k_force<L, PROP>, lds);
+    hipLaunchKernelGGL((k_force<L, PROP>), dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx),
+                       nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->kidx.p,
+                       ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ctx->Gbuf.p, estride, dE);
+}
+
+#define DISPATCH_L(L_, FN, ...)                          \
+    switch (L_) {                                        \
+        case 0: FN<0>(__VA_ARGS__); break;               \
+        case 1: FN<1>(__VA_ARGS__); break;               \
+        case 2: FN<2>(__VA_ARGS__); break;               \
+        case 3: FN<3>(__VA_ARGS__); break;               \
+        case 4: FN<4>(__VA_ARGS__); break;               \
+        case 5: FN<5>(__VA_ARGS__); break;               \
+        case 6: FN<6>(__VA_ARGS__); break;               \
+        case 7: FN<7>(__VA_ARGS__); break;               \
+        default: FN<8>(__VA_ARGS__); break;              \
+    }
+
+template <int L>
+static void launch_force_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
+                             const double* xyz, int64_t nnbr, const int8_t* species,
+                             int64_t estride, double* dE) {
+    if (ctx->prop == ACE_PROP_INVARIANT)
+        launch_force<L, 0>(ctx, st, nenv, off, xyz, nnbr, species, estride, dE);
+    else
+        launch_force<L, 1>(ctx, st, nenv, off, xyz, nnbr, species, estride, dE);
+}
+
+static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                    const int8_t* species, double* E_out, double* dE, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
+    if (nenv == 0) return 0;
+    if (!E_out) return set_err("null E_out");
+    hipStream_t st = (hipStream_t)stream;
+    const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
+    const int64_t ntile = (nenv + 63) / 64;
+    const int64_t estride = ntile * 64;
+    if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
+    const bool forces = dE != nullptr;
+    if (forces && ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * NC * estride)) return -1;
+    WaveCfg* wc = ctx->wcfg[ctx->cur_w];
+    HIP_TRY(hipEventRecord(ctx->ev[0], st));
+    DISPATCH_L(ctx->L, launch_pool, ctx, st, nenv, off, xyz, nnbr, species, estride);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    DagArgs a;
+    a.runs = ctx->runs.p;
+    a.leaf_fid = ctx->leaf_fid.p;
+    a.leaf_coef = ctx->leaf_coef.p;
+    a.tasks = ctx->d_tasks.p;
+    a.task_const = ctx->task_const.p;
+    a.wtask_ptr = forces ? wc->d_ptr_all.p : wc->d_ptr_e.p;
+    a.wtask_list = forces ? wc->d_list_all.p : wc->d_list_e.p;
+    a.npool = ctx->npool;
+    a.W = wc->W;
+    size_t lds = (size_t)ctx->npool * 64 * sizeof(double2);
+    lds = std::max(lds, (size_t)wc->W * NC * 64 * sizeof(double));
+    if (lds > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
+    if (ctx->prop == ACE_PROP_INVARIANT) {
+        set_func_lds((const void*)k_dag<0>, lds);
+        hipLaunchKernelGGL(k_dag<0>, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
+                           ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r, ctx->e0v.p);
+    } else {
+        set_func_lds((const void*)k_dag<1>, lds);
+        hipLaunchKernelGGL(k_dag<1>, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
+                           ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r, ctx->e0v.p);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    if (forces && nnbr > 0) {
+        DISPATCH_L(ctx->L, launch_force_any, ctx, st, nenv, off, xyz, nnbr, species, estride, dE);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    ctx->timing_valid = true;
+    return 0;
+}
+
+extern "C" int ace_energy(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                          const int8_t* species, double* E_out, void* stream) {
+    return run_path(ctx, nenv, off, xyz, species, E_out, nullptr, stream);
+}
+
+extern "C" int ace_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t* off,
+                                 const double* xyz, const int8_t* species, double* E_out,
+                                 double* dE_dr, void* stream) {
+    if (!dE_dr) return set_err("null dE_dr");
+    return run_path(ctx, nenv, off, xyz, species, E_out, dE_dr, stream);
+}
+
+extern "C" int ace_last_timing(ace_ctx* ctx, double* dag_ms, double* pool_ms, double* force_ms) {
+    if (!ctx) return set_err("null context");
+    if (!ctx->timing_valid) return set_err("no timed call yet");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipEventSynchronize(ctx->ev[3]));
+    float a = 0, b = 0, c = 0;
+    HIP_TRY(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
+    if (pool_ms) *pool_ms = a;
+    if (dag_ms) *dag_ms = b;
+    if (force_ms) *force_ms = c;
+    return 0;
+}
+
+// ---- basis level
+
+static int basis_A_internal(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                            const int8_t* species, int64_t nnbr, hipStream_t st) {
+    const int64_t estride = ((nenv + 63) / 64) * 64;
+    if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
+    if (ctx->Afull.ensure((size_t)nenv * ctx->nA)) return -1;
+    DISPATCH_L(ctx->L, launch_pool, ctx, st, nenv, off, xyz, nnbr, species, estride);
+    HIP_TRY(hipGetLastError());
+    const int64_t n = nenv * ctx->nA;
+    hipLaunchKernelGGL(k_expandA, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nenv, ctx->nA,
+                       ctx->aspec_fid.p, ctx->aspec_sgn.p, ctx->Aplus.p, estride, ctx->Afull.p);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ace_basis_A(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                           const int8_t* species, double* A_out, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
+    if (nenv == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
+    HIP_TRY(hipMemcpyAsync(A_out, ctx->Afull.p, sizeof(double2) * nenv * ctx->nA,
+                           hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+static int basis_AA_internal(ace_ctx* ctx, int64_t nenv, hipStream_t st) {
+    if (ctx->AA.ensure((size_t)nenv * ctx->nAA)) return -1;
+    const int64_t n = nenv * ctx->nAA;
+    hipLaunchKernelGGL(k_AA, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nenv, ctx->nA, ctx->nAA,
+                       ctx->maxorder, ctx->orders.p, ctx->iAA2iA.p, ctx->Afull.p, ctx->AA.p);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_realpart(int64_t n, const double2* __restrict__ in, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i].x;
+}
+
+extern "C" int ace_basis_AA(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                            const int8_t* species, double* AA_out, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
+    if (nenv == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
+    if (basis_AA_internal(ctx, nenv, st)) return -1;
+    const int64_t n = nenv * ctx->nAA;
+    if (ctx->prop == ACE_PROP_INVARIANT) {
+        hipLaunchKernelGGL(k_realpart, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, ctx->AA.p, AA_out);
+    } else {
+        HIP_TRY(hipMemcpyAsync(AA_out, ctx->AA.p, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                           const int8_t* species, double* B_out, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
+    if (nenv == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
+    if (basis_AA_internal(ctx, nenv, st)) return -1;
+    const int64_t n = nenv * ctx->nB;
+    if (ctx->prop == ACE_PROP_INVARIANT)
+        hipLaunchKernelGGL(k_B<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nenv, ctx->nB, ctx->nAA,
+                           ctx->rowptr.p, ctx->cols.p, ctx->vals.p, ctx->AA.p, B_out);
+    else
+        hipLaunchKernelGGL(k_B<1>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nenv, ctx->nB, ctx->nAA,
+                           ctx->rowptr.p, ctx->cols.p, ctx->vals.p, ctx->AA.p, B_out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <int L>
+static void launch_dphi(ace_ctx* ctx, hipStream_t st, int64_t nnbr, const double* xyz,
+                        const int8_t* species) {
+    hipLaunchKernelGGL(k_dphi<L>, dim3((unsigned)((nnbr + 63) / 64)), dim3(64), 0, st, radial_params(ctx),
+                       nnbr, ctx->nbr_env.p, xyz, ctx->nspecies > 0 ? species : nullptr, ctx->aspec4.p,
+                       ctx->nA, ctx->dphi.p);
+}
+
+extern "C" int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                            const int8_t* species, double* dB_out, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
+    if (nenv == 0 || nnbr == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
+    if (ctx->nbr_env.ensure(nnbr) || ctx->dphi.ensure((size_t)nnbr * ctx->nA * 3) ||
+        ctx->dAA.ensure((size_t)nnbr * ctx->nAA * 3))
+        return -1;
+    hipLaunchKernelGGL(k_nbr_env, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, off,
+                       ctx->nbr_env.p);
+    DISPATCH_L(ctx->L, launch_dphi, ctx, st, nnbr, xyz, species);
+    HIP_TRY(hipGetLastError());
+    const int64_t n1 = nnbr * ctx->nAA;
+    hipLaunchKernelGGL(k_dAA, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, nnbr, ctx->nA, ctx->nAA,
+                       ctx->maxorder, ctx->nbr_env.p, ctx->orders.p, ctx->iAA2iA.p, ctx->Afull.p,
+                       ctx->dphi.p, ctx->dAA.p);
+    const int64_t n2 = nnbr * ctx->nB;
+    if (ctx->prop == ACE_PROP_INVARIANT)
+        hipLaunchKernelGGL(k_dB<0>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, nnbr, ctx->nB,
+                           ctx->nAA, ctx->rowptr.p, ctx->cols.p, ctx->vals.p, ctx->dAA.p, dB_out);
+    else
+        hipLaunchKernelGGL(k_dB<1>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, nnbr, ctx->nB,
+                           ctx->nAA, ctx->rowptr.p, ctx->cols.p, ctx->vals.p, ctx->dAA.p, dB_out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,216 @@
+// Host-side (CPU) logic of libacehip.so on the flat basis pack: validation of
+// every table before any device use, the effective coefficients
+// c~ = A2B^T c (evaluator.jl:59-67, symmbasis.jl:267-285) and the program
+// leaf / task coefficients derived from them, and the lookup tables of the
+// kernels.  Plain C++ (no HIP) so it is also built and tested on the CPU.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/acehip.h"
+
+namespace acehost {
+
+struct WaveTable {
+    int W;
+    std::vector<int32_t> ptr_all, list_all, ptr_e, list_e;
+};
+
+// Returns "" when the pack is consistent, else a message.
+inline std::string validate_pack(const ace_pack* pk) {
+    if (!pk) return "null pack";
+    if (pk->abi_version != ACEHIP_ABI_VERSION) return "ABI version mismatch";
+    if (pk->prop != ACE_PROP_INVARIANT && pk->prop != ACE_PROP_EUCLIDEAN_VECTOR) return "unknown property";
+    if (pk->L < 0 || pk->L > 8) return "L must be in [0, 8]";
+    if (pk->nr < 1 || pk->nr > 32) return "nr must be in [1, 32]";
+    if (pk->npool < 1 || pk->npool > 0x7fff) return "npool out of range";
+    if (pk->nspecies < 0 || pk->nspecies > 127) return "nspecies out of range";
+    if (pk->tp < 1 || pk->tp > 16) return "transform power out of range";
+    if (pk->pl < 0 || pk->pr < 0 || pk->pl > 16 || pk->pr > 16) return "envelope powers out of range";
+    if (pk->nA < 0 || pk->nAA < 0 || pk->nB < 0 || pk->a2b_nnz < 0) return "negative table size";
+    if (pk->maxorder < 1 || pk->maxorder > 4) return "maxorder must be in [1, 4]";
+    if (!pk->rA || !pk->rB || !pk->rC || !pk->pool) return "null radial/pool table";
+    for (int k = 0; k < pk->npool; ++k) {
+        const int mu = pk->pool[4 * k], n = pk->pool[4 * k + 1], l = pk->pool[4 * k + 2],
+                  m = pk->pool[4 * k + 3];
+        if (n < 1 || n > pk->nr || l < 0 || l > pk->L || m < 0 || m > l) return "pool entry out of range";
+        if (pk->nspecies > 0 ? (mu < 0 || mu >= pk->nspecies) : (mu != -1)) return "pool species out of range";
+    }
+    for (int i = 0; i < pk->nA; ++i)
+        if (pk->aspec_fid[i] < 0 || (pk->aspec_fid[i] & 0x7fff) >= pk->npool) return "aspec_fid out of range";
+    for (int r = 0; r < pk->nAA; ++r) {
+        if (pk->orders[r] < 0 || pk->orders[r] > pk->maxorder) return "orders out of range";
+        for (int t = 0; t < pk->orders[r]; ++t) {
+            const int v = pk->iAA2iA[r * pk->maxorder + t];
+            if (v < 0 || v >= pk->nA) return "iAA2iA out of range";
+        }
+    }
+    if (!pk->a2b_rowptr || pk->a2b_rowptr[0] != 0 || pk->a2b_rowptr[pk->nB] != pk->a2b_nnz) return "bad A2B rowptr";
+    for (int b = 0; b < pk->nB; ++b)
+        if (pk->a2b_rowptr[b + 1] < pk->a2b_rowptr[b]) return "bad A2B rowptr";
+    for (int z = 0; z < pk->a2b_nnz; ++z)
+        if (pk->a2b_cols[z] < 0 || pk->a2b_cols[z] >= pk->nAA) return "A2B column out of range";
+    if (pk->nruns < 0 || pk->nleaves < 0 || pk->ntasks < 0 || pk->nterms < 0 || pk->ngslots < 0)
+        return "negative program size";
+    for (int r = 0; r < pk->nruns; ++r) {
+        const int32_t* run = pk->runs + 7 * (size_t)r;
+        if (run[0] < 0 || run[0] > 3 || run[1] < 1 || run[5] < 0 || run[6] < 0 ||
+            (int64_t)run[6] + run[5] > pk->nleaves)
+            return "run out of range";
+        for (int q = 0; q < run[0]; ++q)
+            if (run[2 + q] < 0 || (run[2 + q] & 0x7fff) >= pk->npool) return "run factor out of range";
+    }
+    for (int i = 0; i < pk->nleaves; ++i)
+        if (pk->leaf_fid[i] < 0 || (pk->leaf_fid[i] & 0x7fff) >= pk->npool) return "leaf factor out of range";
+    if (pk->leaf_term_ptr[0] != 0) return "bad leaf term table";
+    for (int i = 0; i < pk->nleaves; ++i)
+        if (pk->leaf_term_ptr[i + 1] < pk->leaf_term_ptr[i] || pk->leaf_term_ptr[i + 1] > pk->nterms)
+            return "bad leaf term table";
+    for (int t = 0; t < pk->ntasks; ++t) {
+        const int32_t* tk = pk->tasks + 3 * (size_t)t;
+        if (tk[0] < 0 || tk[1] < tk[0] || tk[1] > pk->nruns || tk[2] < -1 || tk[2] >= pk->ngslots)
+            return "task out of range";
+        if (pk->task_term_ptr[t] < 0 || pk->task_term_ptr[t + 1] < pk->task_term_ptr[t] ||
+            pk->task_term_ptr[t + 1] > pk->nterms)
+            return "bad task term table";
+    }
+    for (int tt = 0; tt < pk->nterms; ++tt)
+        if (pk->term_row[tt] < 0 || pk->term_row[tt] >= pk->nAA) return "term row out of range";
+    if (!pk->gslot_ptr || pk->gslot_ptr[0] != 0) return "bad gslot table";
+    for (int k = 0; k < pk->npool; ++k)
+        if (pk->gslot_ptr[k + 1] < pk->gslot_ptr[k]) return "bad gslot table";
+    if (pk->gslot_ptr[pk->npool] > pk->ngslots) return "bad gslot table";
+    for (int s = 0; s < pk->gslot_ptr[pk->npool]; ++s)
+        if (pk->gslot_list[s] < 0 || pk->gslot_list[s] >= pk->ngslots) return "gslot out of range";
+    if (pk->nwave_cfg < 1) return "no wave configuration";
+    return "";
+}
+
+// Parse and check the per-tile-shape task assignments.
+inline std::string wave_tables(const ace_pack* pk, std::vector<WaveTable>& out) {
+    out.clear();
+    size_t pa = 0, la = 0, pe = 0, le = 0;
+    for (int q = 0; q < pk->nwave_cfg; ++q) {
+        const int W = pk->wave_cfg_W[q];
+        if (W < 1 || W > 16) return "waves per tile must be in [1, 16]";
+        WaveTable wt;
+        wt.W = W;
+        wt.ptr_all.assign(pk->wave_task_ptr + pa, pk->wave_task_ptr + pa + W + 1);
+        wt.ptr_e.assign(pk->wave_etask_ptr + pe, pk->wave_etask_ptr + pe + W + 1);
+        if (wt.ptr_all[0] != 0 || wt.ptr_e[0] != 0) return "bad wave task table";
+        for (int w = 0; w < W; ++w)
+            if (wt.ptr_all[w + 1] < wt.ptr_all[w] || wt.ptr_e[w + 1] < wt.ptr_e[w]) return "bad wave task table";
+        const int na = wt.ptr_all[W], ne = wt.ptr_e[W];
+        wt.list_all.assign(pk->wave_task_list + la, pk->wave_task_list + la + na);
+        wt.list_e.assign(pk->wave_etask_list + le, pk->wave_etask_list + le + ne);
+        for (int x : wt.list_all)
+            if (x < 0 || x >= pk->ntasks) return "bad wave task table";
+        for (int x : wt.list_e)
+            if (x < 0 || x >= pk->ntasks) return "bad wave task table";
+        pa += W + 1;
+        pe += W + 1;
+        la += na;
+        le += ne;
+        out.push_back(std::move(wt));
+    }
+    return "";
+}
+
+// Coefficient state derived from c (all host vectors).
+struct Coeffs {
+    std::vector<double> leaf;   // [nleaves][ncoef]
+    std::vector<double> task;   // [ntasks][2 | 6]
+    double e0[6] = {0, 0, 0, 0, 0, 0};  // constant (order-0 AA rows), complex components
+};
+
+// The parts of the pack kept after ace_ctx_create for ace_set_params.
+struct PackCopy {
+    int prop = 0, nAA = 0, nB = 0, nleaves = 0, ntasks = 0;
+    std::vector<int32_t> rowptr, cols, leaf_term_ptr, task_term_ptr, term_row, term_conj, orders;
+    std::vector<double> vals, term_scale;
+    int ncoef() const { return prop == ACE_PROP_INVARIANT ? 1 : 6; }
+};
+
+inline PackCopy copy_pack(const ace_pack* pk) {
+    PackCopy p;
+    p.prop = pk->prop;
+    p.nAA = pk->nAA;
+    p.nB = pk->nB;
+    p.nleaves = pk->nleaves;
+    p.ntasks = pk->ntasks;
+    const int nc = p.ncoef();
+    p.rowptr.assign(pk->a2b_rowptr, pk->a2b_rowptr + pk->nB + 1);
+    p.cols.assign(pk->a2b_cols, pk->a2b_cols + pk->a2b_nnz);
+    p.vals.assign(pk->a2b_vals, pk->a2b_vals + (size_t)pk->a2b_nnz * nc);
+    p.leaf_term_ptr.assign(pk->leaf_term_ptr, pk->leaf_term_ptr + pk->nleaves + 1);
+    p.task_term_ptr.assign(pk->task_term_ptr, pk->task_term_ptr + pk->ntasks + 1);
+    p.term_row.assign(pk->term_row, pk->term_row + pk->nterms);
+    p.term_scale.assign(pk->term_scale, pk->term_scale + pk->nterms);
+    p.term_conj.assign(pk->term_conj, pk->term_conj + pk->nterms);
+    p.orders.assign(pk->orders, pk->orders + pk->nAA);
+    return p;
+}
+
+// c~ = A2B^T c, then every leaf / task coefficient as a sum of its terms.
+inline Coeffs coefficients(const PackCopy& p, const double* c) {
+    const int nc = p.ncoef();
+    std::vector<double> ct((size_t)p.nAA * nc, 0.0);
+    for (int b = 0; b < p.nB; ++b)
+        for (int z = p.rowptr[b]; z < p.rowptr[b + 1]; ++z)
+            for (int q = 0; q < nc; ++q) ct[(size_t)p.cols[z] * nc + q] += p.vals[(size_t)z * nc + q] * c[b];
+    auto add_term = [&](int tt, double* out) {
+        const int row = p.term_row[tt];
+        const double s = p.term_scale[tt];
+        const bool cj = p.term_conj[tt] != 0;
+        for (int q = 0; q < nc; ++q) {
+            double v = ct[(size_t)row * nc + q] * s;
+            if (cj && (q & 1)) v = -v;
+            out[q] += v;
+        }
+    };
+    Coeffs out;
+    out.leaf.assign((size_t)p.nleaves * nc, 0.0);
+    for (int i = 0; i < p.nleaves; ++i)
+        for (int tt = p.leaf_term_ptr[i]; tt < p.leaf_term_ptr[i + 1]; ++tt) add_term(tt, &out.leaf[(size_t)i * nc]);
+    const int tcn = p.prop == ACE_PROP_INVARIANT ? 2 : 6;
+    out.task.assign((size_t)p.ntasks * tcn, 0.0);
+    for (int t = 0; t < p.ntasks; ++t) {
+        double tmp[6] = {0, 0, 0, 0, 0, 0};
+        for (int tt = p.task_term_ptr[t]; tt < p.task_term_ptr[t + 1]; ++tt) add_term(tt, tmp);
+        for (int q = 0; q < nc; ++q) out.task[(size_t)t * tcn + q] = tmp[q];
+    }
+    for (int r = 0; r < p.nAA; ++r)
+        if (p.orders[r] == 0)
+            for (int q = 0; q < nc; ++q) out.e0[q] += ct[(size_t)r * nc + q];
+    return out;
+}
+
+// (mu, l, m, n) -> pooled index, -1 when absent (force kernel lookup).
+inline std::vector<int16_t> pool_lookup(const ace_pack* pk) {
+    const int ns = std::max(1, (int)pk->nspecies);
+    std::vector<int16_t> kx((size_t)ns * (pk->L + 1) * (pk->L + 1) * pk->nr, (int16_t)-1);
+    for (int k = 0; k < pk->npool; ++k) {
+        const int mu = std::max(0, (int)pk->pool[4 * k]), n = pk->pool[4 * k + 1] - 1, l = pk->pool[4 * k + 2],
+                  m = pk->pool[4 * k + 3];
+        kx[(((size_t)mu * (pk->L + 1) + l) * (pk->L + 1) + m) * pk->nr + n] = (int16_t)k;
+    }
+    return kx;
+}
+
+// 1p table as (mu, n, l, m) rows with signed m (basis-level gradient kernel).
+inline std::vector<int32_t> aspec_rows(const ace_pack* pk) {
+    std::vector<int32_t> a4((size_t)pk->nA * 4);
+    for (int i = 0; i < pk->nA; ++i) {
+        const int f = pk->aspec_fid[i], k = f & 0x7fff;
+        a4[4 * i] = pk->pool[4 * k];
+        a4[4 * i + 1] = pk->pool[4 * k + 1];
+        a4[4 * i + 2] = pk->pool[4 * k + 2];
+        a4[4 * i + 3] = (f & 0x8000) ? -pk->pool[4 * k + 3] : pk->pool[4 * k + 3];
+    }
+    return a4;
+}
+
+}  // namespace acehost

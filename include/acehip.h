@@ -1,0 +1,180 @@
+/*
+ * acehip.h — C ABI of libacehip.so, the MI355X (gfx950) basis-evaluation
+ * path of the Atomic Cluster Expansion with ACE.jl (v0.12.46) semantics.
+ *
+ * The library replaces the evaluator slot of ACE.jl's LinearACEModel
+ * (src/linearmodel.jl:44-56: ``evaluator = :standard`` -> ProductEvaluator,
+ * src/evaluator.jl:9-13) and the basis-level evaluate / evaluate_d of
+ * Product1pBasis, PIBasis and SymmetricBasis.  A host (Julia ``ccall``,
+ * Python ``ctypes``) builds the basis on the CPU, hands the flat tables to
+ * ``ace_ctx_create`` and then calls the batched entry points below.
+ *
+ * Conventions
+ *  - Return value 0 = success, < 0 = error; ``ace_last_error()`` gives the
+ *    message (thread-local).  Errors map to the reference's exceptions.
+ *  - All FP64.  Complex values are interleaved (re, im).
+ *  - Environments are ragged: ``env_offsets[nenv+1]`` (int32, ascending,
+ *    env_offsets[0] == 0) index neighbour arrays.  Neighbour positions are
+ *    SoA ``xyz[3][nnbr]`` (x block, then y, then z), relative to the centre.
+ *  - Every array argument of an evaluation call is a DEVICE pointer
+ *    (HBM-resident, e.g. a torch or hipMalloc buffer); ``stream`` is a
+ *    hipStream_t (NULL = default stream).  Calls are stream-ordered and
+ *    return without synchronizing.  The library keeps no caller pointer past
+ *    the call.  One context must not be used by two calls at the same time.
+ *  - ``ace_ctx_create`` / ``ace_set_params`` take HOST pointers.
+ */
+#ifndef ACEHIP_H
+#define ACEHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACEHIP_ABI_VERSION 1
+
+typedef struct ace_ctx ace_ctx;
+
+/* Property of the symmetric basis: Invariant (properties.jl:77-152) or
+ * EuclideanVector (properties.jl:165-248). */
+enum { ACE_PROP_INVARIANT = 0, ACE_PROP_EUCLIDEAN_VECTOR = 1 };
+
+/*
+ * Flat basis definition (host memory; copied by ace_ctx_create).
+ *
+ * Radial basis (orthpolys.jl:82-99, distancetransforms.jl:73):
+ *   t = ((1 + r0) / (1 + r))^tp, P1 = rA[0] fcut(t),
+ *   P2 = (rA[1] t + rB[1]) P1, Pn = (rA[n] t + rB[n]) P(n-1) + rC[n] P(n-2),
+ *   fcut = (t - tl)^pl (t - tr)^pr, zero for t < tl (pl > 0) / t > tr (pr > 0).
+ * Pooled 1p table: pool[k] = (mu, n, l, m) with m >= 0, n 1-based; the 1p
+ *   value A+[k] = sum_j [mu_j == mu] R_n(r_j) Y_l^m(rhat_j).  mu = -1 when
+ *   the 1p basis has no categorical component.
+ * 1p table of the basis: aspec_fid[iA] = k | 0x8000 * (m < 0) and
+ *   aspec_sgn[iA] = (-1)^m for m < 0 (Y_l^-m = (-1)^m conj Y_l^m).
+ * AA table (PIBasisSpec, pibasis.jl:10-13): orders[nAA], iAA2iA[nAA][maxorder]
+ *   (0-based A indices, descending, -1 padding).
+ * A2B map (SymmetricBasis.A2Bmap) in CSR: a2b_rowptr[nB+1], a2b_cols[nnz],
+ *   a2b_vals[nnz * (1 | 6)] (real, or 3 complex for EuclideanVector).
+ * Programs: run programs for P = sum c~ AA and its gradient polynomials
+ *   G[k] (see ace.jl_amd/acehip/programs.py); leaf coefficients are linear in
+ *   c~: leaf_terms[leaf] = (term_begin) into term arrays (term_row,
+ *   term_scale, term_conj); task_terms likewise for task constants.
+ */
+typedef struct ace_pack {
+    int32_t abi_version;
+    int32_t prop;
+    int32_t L;
+    int32_t nr;
+    int32_t nspecies;
+    int32_t tp;
+    double r0, tl, tr;
+    int32_t pl, pr;
+    const double* rA;
+    const double* rB;
+    const double* rC;
+    int32_t npool;
+    const int32_t* pool;          /* [npool][4] */
+    int32_t nA;
+    const int32_t* aspec_fid;     /* [nA] */
+    const double* aspec_sgn;      /* [nA] */
+    int32_t nAA, maxorder;
+    const int32_t* orders;        /* [nAA] */
+    const int32_t* iAA2iA;        /* [nAA][maxorder] */
+    int32_t nB, a2b_nnz;
+    const int32_t* a2b_rowptr;    /* [nB+1] */
+    const int32_t* a2b_cols;      /* [nnz] */
+    const double* a2b_vals;       /* [nnz * ncoef] */
+    /* run programs */
+    int32_t nruns;
+    const int32_t* runs;          /* [nruns][7] depth,jstart,f1,f2,f3,nleaf,leaf_off */
+    int32_t nleaves;
+    const int32_t* leaf_fid;      /* [nleaves] */
+    const int32_t* leaf_term_ptr; /* [nleaves+1] */
+    int32_t ntasks;
+    const int32_t* tasks;         /* [ntasks][3] run_begin, run_end, out_slot (-1 energy) */
+    const int32_t* task_term_ptr; /* [ntasks+1] constant terms */
+    int32_t nterms;
+    const int32_t* term_row;      /* [nterms] AA row of c~ */
+    const double* term_scale;     /* [nterms] */
+    const int32_t* term_conj;     /* [nterms] 1: conj(c~ * scale) */
+    int32_t ngslots;              /* number of G output slots */
+    const int32_t* gslot_ptr;     /* [npool+1] slots summed into G[k] */
+    const int32_t* gslot_list;    /* [ngslots] */
+    /* task -> wave assignment for each supported waves-per-tile setting */
+    int32_t nwave_cfg;
+    const int32_t* wave_cfg_W;    /* [nwave_cfg] waves per workgroup */
+    const int32_t* wave_task_ptr; /* concat over cfgs: [W+1] each (energy+grad) */
+    const int32_t* wave_task_list;/* concat over cfgs: task ids */
+    const int32_t* wave_etask_ptr;/* concat over cfgs: [W+1] each (energy only) */
+    const int32_t* wave_etask_list;
+} ace_pack;
+
+/* Create a context on HIP device ``device`` with coefficients c[nB]
+ * (LinearACEModel(basis, c), linearmodel.jl:44-56; c~ = A2B^T c,
+ * evaluator.jl:30-31,59-67). */
+int ace_ctx_create(const ace_pack* pack, int device, const double* c, int64_t nB,
+                   ace_ctx** out);
+
+/* set_params!(model, c) (linearmodel.jl:67-71 -> evaluator.jl:48-57). */
+int ace_set_params(ace_ctx* ctx, const double* c, int64_t nB);
+
+/* Tile shape: waves per workgroup of the product-DAG kernel (must be one of
+ * the pack's wave configurations). */
+int ace_set_tile(ace_ctx* ctx, int waves_per_tile);
+
+/* Site energies, ProductEvaluator evaluate (evaluator.jl:121-147), batched.
+ * E_out[nenv] (Invariant) or E_out[nenv][3] (EuclideanVector).  species
+ * [nnbr] int8 category index (NULL without a categorical 1p basis). */
+int ace_energy(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
+               const double* xyz, const int8_t* species, double* E_out,
+               void* stream);
+
+/* Energies and grad_config (evaluator.jl:150-200): dE_dr[3][nnbr] SoA is
+ * +dE/dr_j of the environment that owns neighbour j (ACE.jl sign);
+ * EuclideanVector: dE_dr[9][nnbr], entry (p*3+d) = d phi_p / d r_j,d. */
+int ace_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
+                      const double* xyz, const int8_t* species, double* E_out,
+                      double* dE_dr, void* stream);
+
+/* Product1pBasis evaluate (product_1pbasis.jl:123-134): A[nenv][nA] complex. */
+int ace_basis_A(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
+                const double* xyz, const int8_t* species, double* A_out,
+                void* stream);
+
+/* PIBasis evaluate (pibasis.jl:265-275): AA[nenv][nAA], real (Invariant) or
+ * complex (EuclideanVector). */
+int ace_basis_AA(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
+                 const double* xyz, const int8_t* species, double* AA_out,
+                 void* stream);
+
+/* SymmetricBasis evaluate (symmbasis.jl:297-315): B[nenv][nB] (Invariant)
+ * or B[nenv][nB][3] (EuclideanVector). */
+int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
+                const double* xyz, const int8_t* species, double* B_out,
+                void* stream);
+
+/* SymmetricBasis evaluate_d (symmbasis.jl:322-336): dB[nnbr][nB][3]
+ * (Invariant) or dB[nnbr][nB][3][3] (EuclideanVector, [out][space]). */
+int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
+                 const double* xyz, const int8_t* species, double* dB_out,
+                 void* stream);
+
+/* Kernel-time instrumentation: average duration in ms of the last call's
+ * product-DAG kernel (measured with HIP events on the call's stream) and the
+ * executed leaf/run counts per environment. */
+int ace_last_timing(ace_ctx* ctx, double* dag_ms, double* pool_ms, double* force_ms);
+
+/* Host-side validation of env_offsets (non-decreasing, starting at 0) and
+ * species (known categories) before every evaluation call: on (default) or
+ * off for callers that validated their buffers once. */
+int ace_set_validation(ace_ctx* ctx, int on);
+
+const char* ace_last_error(void);
+void ace_ctx_destroy(ace_ctx* ctx);
+int ace_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

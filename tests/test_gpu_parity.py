@@ -1,0 +1,179 @@
+"""GPU parity: libacehip.so through the C ABI vs the CPU oracle.
+
+Tolerances (written here, as the bar): energies / basis values within
+1e-12 relative to the largest value of the batch; forces within 1e-10.
+"""
+import numpy as np
+import pytest
+import torch
+
+import ace_oracle as O
+from acehip import synth
+from acehip.evaluator import EnvBatch, HIPEvaluator
+from helpers import basis_for, random_batch, rel_err
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-12
+F_TOL = 1e-10
+
+
+def _run(basis, T, c, off, R, sp=None, waves=8, forces=True):
+    ev = HIPEvaluator(basis, c, device=0, waves=waves)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    if forces:
+        E, dE = ev.energy_forces(b)
+        torch.cuda.synchronize()
+        return ev, E.cpu().numpy(), dE.cpu().numpy()
+    E = ev.energy(b)
+    torch.cuda.synchronize()
+    return ev, E.cpu().numpy(), None
+
+
+@pytest.mark.parametrize("waves", [4, 8, 16])
+def test_config1_energy_forces(cuda, waves):
+    print(f"[gpu] config1 energy+forces waves={waves}", flush=True)
+    basis, T = basis_for(3, 6, 4)
+    c = synth.coefficients(len(basis), 1)
+    off, R, sp = synth.workload_envs(1, nenv=1000)
+    _, E, dE = _run(basis, T, c, off, R, waves=waves)
+    E_ref = O.energy(T, c, off, R)
+    assert rel_err(E, E_ref) < E_TOL
+    g_ref = O.grad_config(T, c, off, R)
+    assert rel_err(dE.T, g_ref) < F_TOL
+
+
+def test_config1_energy_only(cuda):
+    print("[gpu] config1 energy only", flush=True)
+    basis, T = basis_for(3, 6, 4)
+    c = synth.coefficients(len(basis), 1)
+    off, R, sp = synth.workload_envs(1, nenv=300)
+    _, E, _ = _run(basis, T, c, off, R, forces=False)
+    assert rel_err(E, O.energy(T, c, off, R)) < E_TOL
+
+
+def test_config2_basis_small_batch(cuda):
+    print("[gpu] order-4 degree-12 L6 basis, 48 envs", flush=True)
+    basis, T = basis_for(4, 12, 6)
+    c = synth.coefficients(len(basis), 2)
+    off, R, sp = synth.workload_envs(2, nenv=48)
+    _, E, dE = _run(basis, T, c, off, R)
+    E_ref = O.energy(T, c, off, R)
+    assert rel_err(E, E_ref) < E_TOL
+    g_ref = O.grad_config(T, c, off, R[: off[8]]) if False else None
+    # forces of the first 8 environments (oracle cost)
+    g_ref = O.grad_config(T, c, off[:9], R[: off[8]])
+    assert rel_err(dE.T[: off[8]], g_ref) < F_TOL
+
+
+def test_config3_fcc_forces_fd(cuda):
+    print("[gpu] config3 fcc energy+forces + central FD", flush=True)
+    basis, T = basis_for(3, 6, 4)
+    c = synth.coefficients(len(basis), 3)
+    off, R, _ = synth.workload_envs(3, nenv=64)
+    ev, E, dE = _run(basis, T, c, off, R)
+    assert rel_err(E, O.energy(T, c, off, R)) < E_TOL
+    assert rel_err(dE.T, O.grad_config(T, c, off, R)) < F_TOL
+    # central finite differences of the GPU energy, h = 1e-5
+    h = 1e-5
+    e = 0
+    j0, j1 = off[e], off[e + 1]
+    fd = np.zeros((j1 - j0, 3))
+    for j in range(j0, j1):
+        for d in range(3):
+            Rp = R[j0:j1].copy(); Rp[j - j0, d] += h
+            Rm = R[j0:j1].copy(); Rm[j - j0, d] -= h
+            bp = EnvBatch([0, j1 - j0], Rp.T.copy(), None, device="cuda:0")
+            bm = EnvBatch([0, j1 - j0], Rm.T.copy(), None, device="cuda:0")
+            fd[j - j0, d] = (ev.energy(bp).item() - ev.energy(bm).item()) / (2 * h)
+    assert rel_err(dE.T[j0:j1], fd) < 1e-6
+
+
+def test_vector_property(cuda):
+    print("[gpu] EuclideanVector order 3 degree 6", flush=True)
+    basis, T = basis_for(3, 6, 4, prop="EuclideanVector")
+    c = synth.coefficients(len(basis), 4)
+    off, R, sp = random_batch(40, 11)
+    _, E, dE = _run(basis, T, c, off, R)
+    assert rel_err(E, O.energy(T, c, off, R)) < E_TOL
+    g_ref = O.grad_config(T, c, off, R)              # [nbr, p, d]
+    g = dE.reshape(3, 3, -1).transpose(2, 0, 1)
+    assert rel_err(g, g_ref) < F_TOL
+
+
+def test_two_species_invariant(cuda):
+    print("[gpu] two species invariant order 3 degree 6", flush=True)
+    basis, T = basis_for(3, 6, 3, species=2)
+    c = synth.coefficients(len(basis), 5)
+    off, R, sp = random_batch(40, 12, nspecies=2)
+    _, E, dE = _run(basis, T, c, off, R, sp)
+    assert rel_err(E, O.energy(T, c, off, R, sp)) < E_TOL
+    assert rel_err(dE.T, O.grad_config(T, c, off, R, sp)) < F_TOL
+
+
+def test_basis_level(cuda):
+    print("[gpu] basis-level A, AA, B, dB", flush=True)
+    basis, T = basis_for(3, 6, 4)
+    c = synth.coefficients(len(basis), 1)
+    off, R, sp = random_batch(12, 13)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    A = ev.basis_A(b).cpu().numpy()
+    AA = ev.basis_AA(b).cpu().numpy()
+    B = ev.basis_B(b).cpu().numpy()
+    dB = ev.basis_dB(b).cpu().numpy()
+    A_ref = O.eval_A(T, off, R)
+    AA_ref = O.eval_AA(T, A_ref)
+    assert rel_err(A, A_ref) < E_TOL
+    assert rel_err(AA, AA_ref) < E_TOL
+    assert rel_err(B, O.eval_B(T, AA_ref)) < E_TOL
+    _, dAA = O.eval_AA_dAA(T, off, R)
+    assert rel_err(dB, O.eval_dB(T, dAA)) < F_TOL
+
+
+def test_empty_and_ragged(cuda):
+    print("[gpu] empty environments", flush=True)
+    basis, T = basis_for(3, 6, 4)
+    c = synth.coefficients(len(basis), 1)
+    off = np.array([0, 0, 3, 3, 10], dtype=np.int32)
+    _, _, Rr = None, None, random_batch(1, 14, 10, 10)[1]
+    _, E, dE = _run(basis, T, c, off, Rr)
+    ct = O.ctilde(T, c)
+    e0 = ct[0] if T["orders"][0] == 0 else 0.0
+    assert E[0] == pytest.approx(e0, rel=1e-14)
+    assert E[2] == pytest.approx(e0, rel=1e-14)
+    nz = [1, 3]
+    offs = np.array([0, 3, 10], dtype=np.int32)
+    assert rel_err(E[nz], O.energy(T, c, offs, Rr)) < E_TOL
+    assert rel_err(dE.T, O.grad_config(T, c, offs, Rr)) < F_TOL
+
+
+def test_set_params(cuda):
+    print("[gpu] set_params", flush=True)
+    basis, T = basis_for(3, 6, 4)
+    c1 = synth.coefficients(len(basis), 1)
+    c2 = np.random.default_rng(7).normal(size=len(basis))
+    off, R, _ = random_batch(20, 15)
+    ev = HIPEvaluator(basis, c1, device=0)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    ev.set_params(c2)
+    E, dE = ev.energy_forces(b)
+    assert rel_err(E.cpu().numpy(), O.energy(T, c2, off, R)) < E_TOL
+    assert rel_err(dE.cpu().numpy().T, O.grad_config(T, c2, off, R)) < F_TOL
+
+
+def test_invalid_inputs_raise(cuda):
+    print("[gpu] invalid inputs", flush=True)
+    from acehip._lib import AceError
+    basis, T = basis_for(3, 6, 3, species=2)
+    c = synth.coefficients(len(basis), 5)
+    ev = HIPEvaluator(basis, c, device=0)
+    off, R, sp = random_batch(4, 16, nspecies=2)
+    bad = sp.copy(); bad[0] = 5
+    with pytest.raises(AceError, match="species"):
+        ev.energy(EnvBatch(off, R.T.copy(), bad, device="cuda:0"))
+    with pytest.raises(AceError, match="species array required"):
+        ev.energy(EnvBatch(off, R.T.copy(), None, device="cuda:0"))
+    off2 = off.copy(); off2[2] = off2[1] - 1
+    with pytest.raises(AceError, match="non-decreasing"):
+        ev.energy(EnvBatch(off2, R.T.copy(), sp, device="cuda:0"))
