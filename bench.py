@@ -1,0 +1,181 @@
+"""Benchmark: env/s of ACE energy + forces (plain FP64) on one MI355X.
+
+Workload: 100k synthetic environments, neighbour count Poisson(30) clipped to
+[5, 80], radii U[0.8, 5.0] A, isotropic directions; basis order 4, total
+degree 12, Lmax 6, Invariant, Rn on transformed Jacobi polynomials
+(r0 = 1, rcut = 5); FP64.  One step = energy + forces of the whole batch
+(pool -> product DAG -> force kernels), inputs resident in HBM.
+
+Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
+this build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ace.jl_amd"))
+
+METRIC = "env/s energy+forces: plain FP64 vs surrogate at ≤1 meV/atom MAE"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_flops(pack, forces=True):
+    """Per-environment FLOPs of the AA stage as the reference evaluates it
+    (evaluator.jl:137-143 and pibasis.jl:362-390): 6(nu-1)+2 per AA row for
+    the energy; 6 c(nu) + 4 nu per row of order >= 2 for the adjoint with
+    c(2) = 1, c(nu) = 3 nu - 4."""
+    import numpy as np
+    o = np.asarray(pack.orders)
+    o = o[o >= 1]
+    fe = float(np.sum(6 * (o - 1) + 2))
+    if not forces:
+        return fe
+    o2 = o[o >= 2]
+    c = np.where(o2 == 2, 1, 3 * o2 - 4)
+    return fe + float(np.sum(6 * c + 4 * o2))
+
+
+def executed_flops(pack, forces=True):
+    """FLOPs the product-DAG kernel executes per environment: 4 per leaf
+    (real x complex accumulate), 6 per prefix product, 8 per run (complex
+    multiply-accumulate of prefix x leaf sum)."""
+    w = pack.work_counts(forces)
+    return 4.0 * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
+
+
+def cpu_baseline(T, c, off, R, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from ace_cpu import CpuModel
+    import numpy as np
+    m = CpuModel(T, c)
+    ncpu = len(os.sched_getaffinity(0))
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", ncpu)), ncpu)
+    # probe the rate on a small sample, then size the timed sample to the budget
+    n0 = min(256, len(off) - 1)
+    m.eval(off[: n0 + 1], R[: off[n0]], forces=True, nthreads=threads)
+    t = time.perf_counter()
+    m.eval(off[: n0 + 1], R[: off[n0]], forces=True, nthreads=threads)
+    rate = n0 / (time.perf_counter() - t)
+    n = int(min(len(off) - 1, max(n0, rate * budget_s)))
+    t = time.perf_counter()
+    m.eval(off[: n + 1], R[: off[n]], forces=True, nthreads=threads)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "env/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} environments of the same batch, energy+forces, "
+                      f"C++/OpenMP ProductEvaluator port, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nenv", type=int, default=100_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--waves", type=int, default=0, help="tile waves (0 = autotune)")
+    ap.add_argument("--copy-calib-mb", type=int, default=256,
+                    help="size of a plain copy run after the timed region (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from acehip import synth
+    from acehip.autotune import tune
+    from acehip.evaluator import EnvBatch, HIPEvaluator
+    from acehip.tables import basis_tables
+
+    dev = torch.device("cuda:0")
+    w = synth.WORKLOADS[2]
+    basis = synth.make_basis(w)
+    c = synth.coefficients(len(basis), w["seed"])
+    off, R, sp = synth.workload_envs(2, nenv=args.nenv)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), sp, device=dev)
+    E = torch.empty(b.nenv, dtype=torch.float64, device=dev)
+    dE = torch.empty((3, b.nnbr), dtype=torch.float64, device=dev)
+    ev.energy_forces(b, E, dE)            # validated call
+    torch.cuda.synchronize()
+    ev.set_validation(False)              # inputs validated once above
+    if args.waves:
+        ev.set_tile(args.waves)
+        tuned = None
+    else:
+        tuned = tune(ev, b, forces=True)
+    for _ in range(args.warmup):
+        ev.energy_forces(b, E, dE)
+    torch.cuda.synchronize()
+    # timed region: K steps; per-step DAG kernel time from HIP events on the
+    # launch stream (recorded by the library around each kernel)
+    dag_ms, pool_ms, force_ms = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev.energy_forces(b, E, dE)
+        tm = ev.last_timing()             # synchronizes on the step's last event
+        dag_ms.append(tm["dag_ms"])
+        pool_ms.append(tm["pool_ms"])
+        force_ms.append(tm["force_ms"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    value = b.nenv * args.steps / dt
+    # copy kernel of known size for the counter calibration runs
+    if args.copy_calib_mb > 0:
+        nb = args.copy_calib_mb * (1 << 20) // 8
+        src = torch.ones(nb, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+
+    pk = ev.pack
+    f_alg = algorithmic_flops(pk, True)
+    f_exe = executed_flops(pk, True)
+    dag_avg = float(np.mean(dag_ms))
+    achieved = f_alg * b.nenv / (dag_avg * 1e-3) / 1e12
+    achieved_exe = f_exe * b.nenv / (dag_avg * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "env/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], "
+                               "order 4, degree 12, Lmax 6, Invariant, FP64",
+                   "nenv": b.nenv, "nnbr": b.nnbr, "order": 4, "maxdeg": 12, "L": 6,
+                   "len_B": len(basis), "len_AA": len(pk.orders), "npool": len(pk.pool),
+                   "tile_waves": ev.waves,
+                   "autotune_ms": None if tuned is None else {str(k): round(v, 3) for k, v in tuned[1].items()}},
+        "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (no MFMA); MI355X FP64 peak",
+                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "k_dag<0>", "kernel_ms": round(dag_avg, 4),
+                     "flops_per_env_algorithmic": f_alg, "flops_per_env_executed": f_exe,
+                     "executed_tflops": round(achieved_exe, 3),
+                     "executed_frac": round(achieved_exe / FP64_PEAK_TFLOPS, 4),
+                     "pool_ms": round(float(np.mean(pool_ms)), 4),
+                     "force_ms": round(float(np.mean(force_ms)), 4)},
+    }
+    if not args.no_cpu:
+        T = basis_tables(basis)
+        out["cpu_baseline"] = cpu_baseline(T, c, off, R)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+"""Golden vectors (tests/golden/make_golden.py): the oracle and the basis
+construction reproduce them; the device reproduces them (gpu)."""
+import os
+
+import numpy as np
+import pytest
+
+import ace_oracle as O
+from acehip import synth
+from acehip.tables import basis_tables, load_tables
+from helpers import rel_err
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = [("config1", synth.WORKLOADS[1]), ("vector_deg6", dict(order=3, maxdeg=6, L=4, prop="EuclideanVector"))]
+
+
+def _load(name):
+    T = load_tables(os.path.join(HERE, f"{name}_tables.json"))
+    V = np.load(os.path.join(HERE, f"{name}_vectors.npz"), allow_pickle=False)
+    return T, V
+
+
+@pytest.mark.parametrize("name,w", CASES)
+def test_oracle_reproduces_golden(name, w):
+    T, V = _load(name)
+    off, R, c = V["offsets"], V["R"], V["c"]
+    assert rel_err(O.energy(T, c, off, R), V["E"]) < 1e-14
+    assert rel_err(O.grad_config(T, c, off, R), V["g"]) < 1e-13
+    AA = O.eval_AA(T, O.eval_A(T, off, R))
+    assert rel_err(O.eval_B(T, AA), V["B"]) < 1e-14
+
+
+@pytest.mark.parametrize("name,w", CASES)
+def test_construction_reproduces_golden_tables(name, w):
+    T, V = _load(name)
+    T2 = basis_tables(synth.make_basis(w))
+    for k in ("aspec", "orders", "iAA2iA"):
+        assert np.array_equal(np.asarray(T2[k]), np.asarray(T[k])), k
+    for k in ("A", "B", "C"):
+        assert rel_err(T2["rn"][k], T["rn"][k]) < 1e-14
+    # the A2B map comes from SVDs: compare the spanned space through B values
+    off, R, c = V["offsets"], V["R"], V["c"]
+    B1 = O.eval_B(T, O.eval_AA(T, O.eval_A(T, off, R)))
+    B2 = O.eval_B(T2, O.eval_AA(T2, O.eval_A(T2, off, R)))
+    assert B1.shape == B2.shape
+    M1 = B1.reshape(len(off) - 1, -1)
+    M2 = B2.reshape(len(off) - 1, -1)
+    assert abs(np.linalg.matrix_rank(np.vstack([M1, M2])) - np.linalg.matrix_rank(M1)) == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,w", CASES)
+def test_device_reproduces_golden(cuda, name, w):
+    import torch
+    from acehip.evaluator import EnvBatch, HIPEvaluator
+    print(f"[gpu] golden {name}", flush=True)
+    T, V = _load(name)
+    basis = synth.make_basis(w)
+    assert np.array_equal(basis_tables(basis)["iAA2iA"], T["iAA2iA"])
+    # evaluate the stored tables: rebuild the basis object around them
+    basis.A2Bmap.rows = np.asarray(T["a2b"]["rows"])
+    basis.A2Bmap.cols = np.asarray(T["a2b"]["cols"])
+    basis.A2Bmap.vals = np.asarray(T["a2b"]["vals"])
+    off, R, c = V["offsets"], V["R"], V["c"]
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    B = ev.basis_B(b)
+    torch.cuda.synchronize()
+    assert rel_err(E.cpu().numpy(), V["E"]) < 1e-12
+    g = dE.cpu().numpy()
+    g = g.T if g.shape[0] == 3 else g.reshape(3, 3, -1).transpose(2, 0, 1)
+    assert rel_err(g, V["g"]) < 1e-10
+    assert rel_err(B.cpu().numpy(), V["B"]) < 1e-12
