@@ -36,6 +36,7 @@ class AcePack(C.Structure):
         ("ntasks", C.c_int32), ("tasks", _i32p), ("task_term_ptr", _i32p),
         ("nterms", C.c_int32), ("term_row", _i32p), ("term_scale", _f64p), ("term_conj", _i32p),
         ("ngslots", C.c_int32), ("gslot_ptr", _i32p), ("gslot_list", _i32p),
+        ("nrec", C.c_int32), ("rec_hdr", C.POINTER(C.c_uint32)), ("rec_leaf", _i32p),
         ("nwave_cfg", C.c_int32), ("wave_cfg_W", _i32p), ("wave_task_ptr", _i32p),
         ("wave_task_list", _i32p), ("wave_etask_ptr", _i32p), ("wave_etask_list", _i32p),
     ]
@@ -149,6 +150,9 @@ def make_pack(pk):
     s.ngslots = pk.ngslots
     s.gslot_ptr = _ptr(arr(pk.gslot_ptr, np.int32), C.c_int32)
     s.gslot_list = _ptr(arr(pk.gslot_list, np.int32), C.c_int32)
+    s.nrec = len(pk.rec_hdr)
+    s.rec_hdr = _ptr(arr(pk.rec_hdr, np.uint32), C.c_uint32)
+    s.rec_leaf = _ptr(arr(pk.rec_leaf, np.int32), C.c_int32)
     s.nwave_cfg = len(pk.wave_cfg_W)
     s.wave_cfg_W = _ptr(arr(pk.wave_cfg_W, np.int32), C.c_int32)
     s.wave_task_ptr = _ptr(arr(pk.wave_task_ptr, np.int32), C.c_int32)

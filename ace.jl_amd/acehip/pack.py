@@ -14,6 +14,7 @@ from . import programs as P
 
 WAVE_CONFIGS = (4, 8, 16)     # waves per workgroup of the product-DAG kernel
 ENERGY_PARTS = 16             # energy polynomial split into this many tasks
+REC_LEAVES = 4                # leaves per 64-byte program record
 MAX_TASK_COST = 6000.0        # gradient polynomials above this are split
 
 
@@ -59,7 +60,8 @@ class Pack:
         # flatten programs
         self.runs = np.asarray(prog.runs, dtype=np.int32).reshape(-1, 7)
         self.leaf_fid = np.asarray(prog.leaf_fid, dtype=np.int32)
-        self.tasks = np.asarray(prog.tasks, dtype=np.int32).reshape(-1, 3)
+        tasks3 = np.asarray(prog.tasks, dtype=np.int32).reshape(-1, 3)
+        self._build_records(tasks3)
         lt_ptr, rows, scales, conjs = [0], [], [], []
         for terms in prog.leaf_terms:
             for (r, s, c) in terms:
@@ -123,6 +125,38 @@ class Pack:
         self.maxorder = int(iAA2iA.shape[1])
         self.iAA2iA = iAA2iA.astype(np.int32)
 
+    def _build_records(self, tasks3):
+        """64-byte records: 8 header dwords + REC_LEAVES coefficients.
+
+        dw0 = depth | jstart << 2 | nleaf << 5; dw1 = f1 | f2 << 16;
+        dw2 = f3; dw3.. = leaf factor ids, two per dword.  A run with more
+        than REC_LEAVES leaves continues in further records with the same
+        prefix (jstart = depth + 1).  ``rec_leaf[r, i]`` is the leaf whose
+        coefficient fills slot i (-1: empty)."""
+        K = REC_LEAVES
+        hdr, rleaf = [], []
+        rec_of_task = []
+        for (r0, r1, slot) in tasks3:
+            b = len(hdr)
+            for r in range(r0, r1):
+                d, j, f1, f2, f3, nl, lo = (int(x) for x in self.runs[r])
+                for q0 in range(0, max(nl, 1), K):
+                    n = min(K, nl - q0)
+                    jj = j if q0 == 0 else d + 1
+                    h = [0] * 8
+                    h[0] = d | (jj << 2) | (n << 5)
+                    h[1] = f1 | (f2 << 16)
+                    h[2] = f3
+                    ids = [int(self.leaf_fid[lo + q0 + i]) if i < n else 0 for i in range(K)]
+                    for i in range(0, K, 2):
+                        h[3 + i // 2] = ids[i] | (ids[i + 1] << 16)
+                    hdr.append(h)
+                    rleaf.append([lo + q0 + i if i < n else -1 for i in range(K)])
+            rec_of_task.append((b, len(hdr)))
+        self.rec_hdr = np.asarray(hdr, dtype=np.uint32).reshape(-1, 8)
+        self.rec_leaf = np.asarray(rleaf, dtype=np.int32).reshape(-1, K)
+        self.tasks = np.concatenate([tasks3, np.asarray(rec_of_task, dtype=np.int32).reshape(-1, 2)], axis=1)
+
     # -- statistics used by the roofline accounting
     def work_counts(self, forces: bool):
         """Executed leaf and run counts per environment of the product-DAG
@@ -131,9 +165,10 @@ class Pack:
         leaves = runs = 0
         newf = 0
         for t in tasks:
-            r0, r1, _ = self.tasks[t]
+            r0, r1 = self.tasks[t][0], self.tasks[t][1]
             rr = self.runs[r0:r1]
             leaves += int(rr[:, 5].sum())
             runs += len(rr)
             newf += int(np.sum(np.maximum(rr[:, 0] - rr[:, 1] + 1, 0)))
-        return {"leaves": leaves, "runs": runs, "prefix_mults": newf}
+        recs = sum(int(self.tasks[t][4] - self.tasks[t][3]) for t in tasks)
+        return {"leaves": leaves, "runs": runs, "prefix_mults": newf, "records": recs}

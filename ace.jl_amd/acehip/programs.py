@@ -168,7 +168,7 @@ class RunProgram:
         return len(self.tasks) - 1
 
     def cost(self, t, leaf_w=1.0, run_w=2.0):
-        r0, r1, _ = self.tasks[t]
+        r0, r1 = self.tasks[t][0], self.tasks[t][1]
         return leaf_w * sum(self.runs[r][5] for r in range(r0, r1)) + run_w * (r1 - r0)
 
 

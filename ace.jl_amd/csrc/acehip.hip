@@ -183,6 +183,16 @@ __device__ __forceinline__ void alp_eval_ed(const Sph& S, double* P, double* dP)
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
+// Give every XCD a contiguous range of environments so that neighbouring
+// environments, which share cache lines of the [k][env] arrays, meet in the
+// same L2.  Bijection on [0, n).
+__device__ __forceinline__ int64_t xcd_env(int64_t b, int64_t n) {
+    const int64_t x = b & 7, i = b >> 3;
+    const int64_t q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // ---------------------------------------------------------------------------
 // k_pool: A+[k][env]
 
@@ -196,7 +206,7 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
     constexpr int NP = (L + 1) * (L + 2) / 2;
     extern __shared__ double smem[];
     const int lane = threadIdx.x;
-    const int64_t env = blockIdx.x;
+    const int64_t env = xcd_env(blockIdx.x, nenv);
     const int nr = rp.nr;
     const int rstride = nr + 2 * NP + 1;  // doubles per neighbour slot
     double* base = smem;
@@ -299,38 +309,46 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+template <int ENVS>
 __device__ __forceinline__ double2 lds_factor(const double2* As, int f, int lane) {
-    double2 v = As[(f & 0x7fff) * 64 + lane];
+    double2 v = As[(f & 0x7fff) * ENVS + lane];
     if (f & ACE_CONJ) v.y = -v.y;
     return v;
 }
 
 // PROP 0: real coefficient, output Re(acc) (energy) or acc (G slot).
 // PROP 1: 3 complex coefficients, output Re(acc_p) or acc_p.
-template <int PROP>
+// ENVS = 64: lanes = 64 environments, one task stream per wave.
+// ENVS = 32: lanes 0-31 and 32-63 are the same 32 environments running two
+//            task streams (halves the LDS tile for large pooled bases).
+template <int PROP, int ENVS>
 __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
                                               const double2* __restrict__ Aplus,
                                               int64_t estride, double* __restrict__ E_out,
                                               double2* __restrict__ Gbuf, double e0r,
                                               const double* __restrict__ e0v) {
     constexpr int NC = PROP == 0 ? 1 : 3;
-    extern __shared__ double2 As[];  // [npool][64]
+    constexpr int SPW = 64 / ENVS;  // task streams per wave
+    extern __shared__ double2 As[];  // [npool][ENVS]
     const int lane = threadIdx.x & 63;
+    const int el = lane & (ENVS - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t env = (int64_t)blockIdx.x * 64 + lane;
+    const int nwaves = blockDim.x >> 6;
+    const int stream = wave * SPW + lane / ENVS;
+    const int64_t env = (int64_t)blockIdx.x * ENVS + el;
     const bool valid = env < nenv;
-    for (int k = wave; k < a.npool; k += a.W)
-        As[k * 64 + lane] = valid ? Aplus[(int64_t)k * estride + env] : make_double2(0.0, 0.0);
+    for (int k = wave * SPW + lane / ENVS; k < a.npool; k += nwaves * SPW)
+        As[k * ENVS + el] = valid ? Aplus[(int64_t)k * estride + env] : make_double2(0.0, 0.0);
     __syncthreads();
 
     double ep[NC];
 #pragma unroll
     for (int p = 0; p < NC; ++p) ep[p] = 0.0;
 
-    const int tb = a.wtask_ptr[wave], te = a.wtask_ptr[wave + 1];
+    const int tb = a.wtask_ptr[stream], te = a.wtask_ptr[stream + 1];
     for (int ti = tb; ti < te; ++ti) {
         const int t = a.wtask_list[ti];
-        const int rb = a.tasks[3 * t], re = a.tasks[3 * t + 1], slot = a.tasks[3 * t + 2];
+        const int rb = a.tasks[5 * t], re = a.tasks[5 * t + 1], slot = a.tasks[5 * t + 2];
         double2 acc[NC];
 #pragma unroll
         for (int p = 0; p < NC; ++p)
@@ -341,16 +359,16 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
             const int* run = a.runs + 7 * r;
             const int d = run[0], j = run[1];
             const int nl = run[5], lo = run[6];
-            if (j <= 1 && d >= 1) P1 = lds_factor(As, run[2], lane);
-            if (j <= 2 && d >= 2) P2 = cmul(P1, lds_factor(As, run[3], lane));
-            if (j <= 3 && d >= 3) P3 = cmul(P2, lds_factor(As, run[4], lane));
+            if (j <= 1 && d >= 1) P1 = lds_factor<ENVS>(As, run[2], el);
+            if (j <= 2 && d >= 2) P2 = cmul(P1, lds_factor<ENVS>(As, run[3], el));
+            if (j <= 3 && d >= 3) P3 = cmul(P2, lds_factor<ENVS>(As, run[4], el));
             const double2 Pd = d == 0 ? make_double2(1.0, 0.0) : d == 1 ? P1 : d == 2 ? P2 : P3;
             double2 S[NC];
 #pragma unroll
             for (int p = 0; p < NC; ++p) S[p] = make_double2(0.0, 0.0);
             for (int i = 0; i < nl; ++i) {
                 const int f = a.leaf_fid[lo + i];
-                const double2 v = lds_factor(As, f, lane);
+                const double2 v = lds_factor<ENVS>(As, f, el);
                 if (PROP == 0) {
                     const double c = a.leaf_coef[lo + i];
                     S[0].x += c * v.x;
@@ -378,19 +396,128 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
             for (int p = 0; p < NC; ++p) Gbuf[((int64_t)slot * NC + p) * estride + env] = acc[p];
         }
     }
-    // reduce energy partials over waves (reuse the A tile region)
+    // reduce energy partials over streams (reuse the A tile region)
     __syncthreads();
     double* red = reinterpret_cast<double*>(As);
 #pragma unroll
-    for (int p = 0; p < NC; ++p) red[(wave * NC + p) * 64 + lane] = ep[p];
+    for (int p = 0; p < NC; ++p) red[(stream * NC + p) * ENVS + el] = ep[p];
     __syncthreads();
-    if (wave == 0 && valid && E_out) {
+    if (wave == 0 && lane < ENVS && valid && E_out) {
 #pragma unroll
         for (int p = 0; p < NC; ++p) {
-            double s = PROP == 0 ? e0r : e0v[p];
-            for (int w = 0; w < a.W; ++w) s += red[(w * NC + p) * 64 + lane];
-            E_out[env * NC + p] = s;
+            double sum = PROP == 0 ? e0r : e0v[p];
+            for (int w = 0; w < nwaves * SPW; ++w) sum += red[(w * NC + p) * ENVS + el];
+            E_out[env * NC + p] = sum;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_dagr: the same programs as 64-byte records (Invariant).  A record holds
+// the run header and up to four leaves with their coefficients; the next
+// record is loaded while the current record's LDS reads are in flight.
+
+struct DagRecArgs {
+    const uint32_t* recs;      // [nrec + 1][16]
+    const int32_t* tasks;      // [ntasks][5]
+    const double* task_const;  // [ntasks][2]
+    const int32_t* wtask_ptr;  // [W+1]
+    const int32_t* wtask_list;
+    int npool;
+    int W;
+};
+
+__device__ __forceinline__ double flip_sign_if(double c, uint32_t f) {
+    // negate c when the conjugation bit (0x8000) of factor id f is set
+    return __hiloint2double(__double2hiint(c) ^ (int)((f & 0x8000u) << 16), __double2loint(c));
+}
+
+__device__ __forceinline__ double2 lds_conj(const double2* As, uint32_t f, int lane) {
+    double2 v = As[(f & 0x7fffu) * 64 + lane];
+    v.y = flip_sign_if(v.y, f);
+    return v;
+}
+
+__device__ __forceinline__ double rec_coef(uint32_t lo, uint32_t hi) {
+    return __hiloint2double((int)hi, (int)lo);
+}
+
+__global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
+                                               const double2* __restrict__ Aplus,
+                                               int64_t estride, double* __restrict__ E_out,
+                                               double2* __restrict__ Gbuf, double e0r) {
+    extern __shared__ double2 As[];  // [npool][64]
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t env = (int64_t)blockIdx.x * 64 + lane;
+    const bool valid = env < nenv;
+    for (int k = wave; k < a.npool; k += a.W)
+        As[k * 64 + lane] = valid ? Aplus[(int64_t)k * estride + env] : make_double2(0.0, 0.0);
+    __syncthreads();
+
+    double ep = 0.0;
+    const int tb = a.wtask_ptr[wave], te = a.wtask_ptr[wave + 1];
+    for (int ti = tb; ti < te; ++ti) {
+        const int t = a.wtask_list[ti];
+        const int slot = a.tasks[5 * t + 2], rb = a.tasks[5 * t + 3], re = a.tasks[5 * t + 4];
+        double2 acc = make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
+        double2 P1 = make_double2(1.0, 0.0), P2 = P1, P3 = P1;
+        const uint4* rp = reinterpret_cast<const uint4*>(a.recs) + 4 * (int64_t)rb;
+        uint4 h0 = rp[0], h1 = rp[1], h2 = rp[2], h3 = rp[3];
+        for (int r = rb; r < re; ++r) {
+            // next record (record nrec exists and is all zeros)
+            const uint4* np = reinterpret_cast<const uint4*>(a.recs) + 4 * (int64_t)(r + 1);
+            const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            const uint32_t hd = h0.x;
+            const int d = hd & 3, j = (hd >> 2) & 7, nl = (hd >> 5) & 7;
+            const uint32_t l0 = h0.w & 0xffffu, l1 = h0.w >> 16;
+            // leaf values (slots 0, 1 always; 2, 3 when present)
+            const double2 v0 = As[(l0 & 0x7fffu) * 64 + lane];
+            const double2 v1 = As[(l1 & 0x7fffu) * 64 + lane];
+            const double c0 = rec_coef(h2.x, h2.y), c1 = rec_coef(h2.z, h2.w);
+            double2 S;
+            S.x = c0 * v0.x + c1 * v1.x;
+            S.y = flip_sign_if(c0, l0) * v0.y + flip_sign_if(c1, l1) * v1.y;
+            if (nl > 2) {
+                const uint32_t l2 = h1.x & 0xffffu, l3 = h1.x >> 16;
+                const double2 v2 = As[(l2 & 0x7fffu) * 64 + lane];
+                const double2 v3 = As[(l3 & 0x7fffu) * 64 + lane];
+                const double c2 = rec_coef(h3.x, h3.y), c3 = rec_coef(h3.z, h3.w);
+                S.x += c2 * v2.x + c3 * v3.x;
+                S.y += flip_sign_if(c2, l2) * v2.y + flip_sign_if(c3, l3) * v3.y;
+            }
+            // prefix products for the levels that changed
+            if (j <= d) {
+                if (j <= 1) P1 = lds_conj(As, h0.y & 0xffffu, lane);
+                if (j <= 2 && d >= 2) P2 = cmul(P1, lds_conj(As, h0.y >> 16, lane));
+                if (d >= 3) P3 = cmul(P2, lds_conj(As, h0.z & 0xffffu, lane));
+            }
+            if (d == 0) {
+                acc.x += S.x;
+                acc.y += S.y;
+            } else {
+                const double2 Pd = d == 1 ? P1 : (d == 2 ? P2 : P3);
+                acc.x += Pd.x * S.x - Pd.y * S.y;
+                acc.y += Pd.x * S.y + Pd.y * S.x;
+            }
+            h0 = n0;
+            h1 = n1;
+            h2 = n2;
+            h3 = n3;
+        }
+        if (slot < 0)
+            ep += acc.x;
+        else if (valid)
+            Gbuf[(int64_t)slot * estride + env] = acc;
+    }
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(As);
+    red[wave * 64 + lane] = ep;
+    __syncthreads();
+    if (wave == 0 && valid && E_out) {
+        double s = e0r;
+        for (int w = 0; w < a.W; ++w) s += red[w * 64 + lane];
+        E_out[env] = s;
     }
 }
 
@@ -411,7 +538,7 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, int64_t nenv,
     constexpr int NC = PROP == 0 ? 1 : 3;
     extern __shared__ double smem[];
     const int lane = threadIdx.x;
-    const int64_t env = blockIdx.x;
+    const int64_t env = xcd_env(blockIdx.x, nenv);
     const int nr = rp.nr;
     double2* Gs = reinterpret_cast<double2*>(smem);
     double* Rs = reinterpret_cast<double*>(Gs + npool * NC);  // [64][2*nr]
@@ -788,6 +915,7 @@ struct ace_ctx {
     DevBuf<int16_t> kidx;
     DevBuf<int32_t> runs, leaf_fid, d_tasks, gslot_ptr, gslot_list;
     DevBuf<double> leaf_coef, task_const, e0v;
+    DevBuf<uint32_t> recs;
     double e0r = 0.0;
     std::vector<WaveCfg*> wcfg;
     int cur_w = 0;
@@ -808,6 +936,10 @@ struct ace_ctx {
 static int upload_coeffs(ace_ctx* ctx, const double* c) {
     acehost::Coeffs cf = acehost::coefficients(ctx->host, c);
     if (ctx->leaf_coef.upload(cf.leaf.data(), cf.leaf.size())) return -1;
+    if (ctx->prop == ACE_PROP_INVARIANT) {
+        std::vector<uint32_t> rec = acehost::records(ctx->host, cf);
+        if (ctx->recs.upload(rec.data(), rec.size())) return -1;
+    }
     if (ctx->task_const.upload(cf.task.data(), cf.task.size())) return -1;
     ctx->e0r = cf.e0[0];
     double ev[3] = {cf.e0[0], cf.e0[2], cf.e0[4]};
@@ -897,7 +1029,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         ctx->vals.upload(pk->a2b_vals, (size_t)pk->a2b_nnz * nc) ||
         ctx->runs.upload(pk->runs, 7 * (size_t)pk->nruns) ||
         ctx->leaf_fid.upload(pk->leaf_fid, pk->nleaves) ||
-        ctx->d_tasks.upload(pk->tasks, 3 * (size_t)pk->ntasks) ||
+        ctx->d_tasks.upload(pk->tasks, 5 * (size_t)pk->ntasks) ||
         ctx->gslot_ptr.upload(pk->gslot_ptr, pk->npool + 1) ||
         ctx->gslot_list.upload(pk->gslot_list, pk->gslot_ptr[pk->npool]))
         return fail("upload");
@@ -1086,14 +1218,33 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     a.W = wc->W;
     size_t lds = (size_t)ctx->npool * 64 * sizeof(double2);
     lds = std::max(lds, (size_t)wc->W * NC * 64 * sizeof(double));
-    if (lds > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
+    if (ctx->prop == ACE_PROP_INVARIANT && lds > 160 * 1024)
+        return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     if (ctx->prop == ACE_PROP_INVARIANT) {
-        set_func_lds((const void*)k_dag<0>, lds);
-        hipLaunchKernelGGL(k_dag<0>, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
+        DagRecArgs ra;
+        ra.recs = ctx->recs.p;
+        ra.tasks = ctx->d_tasks.p;
+        ra.task_const = ctx->task_const.p;
+        ra.wtask_ptr = a.wtask_ptr;
+        ra.wtask_list = a.wtask_list;
+        ra.npool = ctx->npool;
+        ra.W = wc->W;
+        set_func_lds((const void*)k_dagr, lds);
+        hipLaunchKernelGGL(k_dagr, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, ra, nenv,
+                           ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r);
+    } else if ((size_t)ctx->npool * 64 * sizeof(double2) <= 160 * 1024) {
+        set_func_lds((const void*)k_dag<1, 64>, lds);
+        hipLaunchKernelGGL((k_dag<1, 64>), dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
                            ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r, ctx->e0v.p);
     } else {
-        set_func_lds((const void*)k_dag<1>, lds);
-        hipLaunchKernelGGL(k_dag<1>, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
+        // 32-environment tiles, two task streams per wave: the W stream
+        // assignment runs on W/2 waves
+        if (wc->W < 2) return set_err("tile configuration needs an even number of streams");
+        const int64_t nt32 = (nenv + 31) / 32;
+        size_t lds32 = std::max((size_t)ctx->npool * 32 * sizeof(double2), (size_t)wc->W * NC * 32 * sizeof(double));
+        if (lds32 > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 320)");
+        set_func_lds((const void*)k_dag<1, 32>, lds32);
+        hipLaunchKernelGGL((k_dag<1, 32>), dim3((unsigned)nt32), dim3(64 * (wc->W / 2)), lds32, st, a, nenv,
                            ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r, ctx->e0v.p);
     }
     HIP_TRY(hipGetLastError());

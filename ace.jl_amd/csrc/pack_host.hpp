@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -69,10 +70,28 @@ inline std::string validate_pack(const ace_pack* pk) {
     for (int i = 0; i < pk->nleaves; ++i)
         if (pk->leaf_term_ptr[i + 1] < pk->leaf_term_ptr[i] || pk->leaf_term_ptr[i + 1] > pk->nterms)
             return "bad leaf term table";
+    if (pk->nrec < 0 || (pk->nrec > 0 && (!pk->rec_hdr || !pk->rec_leaf))) return "bad record table";
+    for (int r = 0; r < pk->nrec; ++r) {
+        const uint32_t* h = pk->rec_hdr + 8 * (size_t)r;
+        const int d = h[0] & 3, j = (h[0] >> 2) & 7, nl = (h[0] >> 5) & 7;
+        if (h[0] >> 8) return "record header out of range";
+        if (j < 1 || j > d + 1 || nl < 1 || nl > 4) return "record header out of range";
+        const uint32_t f[3] = {h[1] & 0xffff, h[1] >> 16, h[2] & 0xffff};
+        for (int q = 0; q < 3; ++q)
+            if ((f[q] & 0x7fff) >= (uint32_t)pk->npool) return "record factor out of range";
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t w = h[3 + i / 2];
+            const uint32_t fid = (i & 1) ? (w >> 16) : (w & 0xffff);
+            if ((fid & 0x7fff) >= (uint32_t)pk->npool) return "record factor out of range";
+            const int lf = pk->rec_leaf[4 * (size_t)r + i];
+            if ((i < nl) != (lf >= 0) || lf >= pk->nleaves) return "record leaf out of range";
+        }
+    }
     for (int t = 0; t < pk->ntasks; ++t) {
-        const int32_t* tk = pk->tasks + 3 * (size_t)t;
+        const int32_t* tk = pk->tasks + 5 * (size_t)t;
         if (tk[0] < 0 || tk[1] < tk[0] || tk[1] > pk->nruns || tk[2] < -1 || tk[2] >= pk->ngslots)
             return "task out of range";
+        if (tk[3] < 0 || tk[4] < tk[3] || tk[4] > pk->nrec) return "task out of range";
         if (pk->task_term_ptr[t] < 0 || pk->task_term_ptr[t + 1] < pk->task_term_ptr[t] ||
             pk->task_term_ptr[t + 1] > pk->nterms)
             return "bad task term table";
@@ -128,7 +147,9 @@ struct Coeffs {
 
 // The parts of the pack kept after ace_ctx_create for ace_set_params.
 struct PackCopy {
-    int prop = 0, nAA = 0, nB = 0, nleaves = 0, ntasks = 0;
+    int prop = 0, nAA = 0, nB = 0, nleaves = 0, ntasks = 0, nrec = 0;
+    std::vector<uint32_t> rec_hdr;
+    std::vector<int32_t> rec_leaf;
     std::vector<int32_t> rowptr, cols, leaf_term_ptr, task_term_ptr, term_row, term_conj, orders;
     std::vector<double> vals, term_scale;
     int ncoef() const { return prop == ACE_PROP_INVARIANT ? 1 : 6; }
@@ -151,6 +172,9 @@ inline PackCopy copy_pack(const ace_pack* pk) {
     p.term_scale.assign(pk->term_scale, pk->term_scale + pk->nterms);
     p.term_conj.assign(pk->term_conj, pk->term_conj + pk->nterms);
     p.orders.assign(pk->orders, pk->orders + pk->nAA);
+    p.nrec = pk->nrec;
+    p.rec_hdr.assign(pk->rec_hdr, pk->rec_hdr + 8 * (size_t)pk->nrec);
+    p.rec_leaf.assign(pk->rec_leaf, pk->rec_leaf + 4 * (size_t)pk->nrec);
     return p;
 }
 
@@ -185,6 +209,23 @@ inline Coeffs coefficients(const PackCopy& p, const double* c) {
     for (int r = 0; r < p.nAA; ++r)
         if (p.orders[r] == 0)
             for (int q = 0; q < nc; ++q) out.e0[q] += ct[(size_t)r * nc + q];
+    return out;
+}
+
+// Device records (Invariant): 8 header dwords + 4 coefficients, plus one
+// zero record at the end so the kernel can always prefetch record r + 1.
+inline std::vector<uint32_t> records(const PackCopy& p, const Coeffs& cf) {
+    std::vector<uint32_t> out(16 * ((size_t)p.nrec + 1), 0u);
+    for (int r = 0; r < p.nrec; ++r) {
+        uint32_t* o = out.data() + 16 * (size_t)r;
+        for (int q = 0; q < 8; ++q) o[q] = p.rec_hdr[8 * (size_t)r + q];
+        double c[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i) {
+            const int lf = p.rec_leaf[4 * (size_t)r + i];
+            if (lf >= 0) c[i] = cf.leaf[lf];
+        }
+        std::memcpy(o + 8, c, sizeof(c));
+    }
     return out;
 }
 

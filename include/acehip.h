@@ -92,7 +92,8 @@ typedef struct ace_pack {
     const int32_t* leaf_fid;      /* [nleaves] */
     const int32_t* leaf_term_ptr; /* [nleaves+1] */
     int32_t ntasks;
-    const int32_t* tasks;         /* [ntasks][3] run_begin, run_end, out_slot (-1 energy) */
+    const int32_t* tasks;         /* [ntasks][5] run_begin, run_end, out_slot (-1 energy),
+                                     rec_begin, rec_end */
     const int32_t* task_term_ptr; /* [ntasks+1] constant terms */
     int32_t nterms;
     const int32_t* term_row;      /* [nterms] AA row of c~ */
@@ -101,6 +102,12 @@ typedef struct ace_pack {
     int32_t ngslots;              /* number of G output slots */
     const int32_t* gslot_ptr;     /* [npool+1] slots summed into G[k] */
     const int32_t* gslot_list;    /* [ngslots] */
+    /* the same programs as 64-byte records: 8 header dwords (dw0 = depth |
+     * jstart << 2 | nleaf << 5, dw1 = f1 | f2 << 16, dw2 = f3, dw3.. leaf
+     * factor ids two per dword) + 4 coefficients filled from rec_leaf */
+    int32_t nrec;
+    const uint32_t* rec_hdr;      /* [nrec][8] */
+    const int32_t* rec_leaf;      /* [nrec][4] leaf index per slot, -1 empty */
     /* task -> wave assignment for each supported waves-per-tile setting */
     int32_t nwave_cfg;
     const int32_t* wave_cfg_W;    /* [nwave_cfg] waves per workgroup */

@@ -177,3 +177,15 @@ def test_invalid_inputs_raise(cuda):
     off2 = off.copy(); off2[2] = off2[1] - 1
     with pytest.raises(AceError, match="non-decreasing"):
         ev.energy(EnvBatch(off2, R.T.copy(), sp, device="cuda:0"))
+
+
+def test_config4_two_species_vector(cuda):
+    print("[gpu] config4: 2 species, EuclideanVector, order 3, degree 10, L 5 (32-env tiles)", flush=True)
+    basis, T = basis_for(3, 10, 5, prop="EuclideanVector", species=2)
+    c = synth.coefficients(len(basis), 4)
+    off, R, sp = synth.workload_envs(4, nenv=6)
+    _, E, dE = _run(basis, T, c, off, R, sp)
+    assert rel_err(E, O.energy(T, c, off, R, sp)) < E_TOL
+    g_ref = O.grad_config(T, c, off[:3], R[: off[2]], sp[: off[2]])
+    g = dE.reshape(3, 3, -1).transpose(2, 0, 1)[: off[2]]
+    assert rel_err(g, g_ref) < F_TOL

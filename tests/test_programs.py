@@ -22,7 +22,7 @@ def emulate(pk, Aplus, ct):
     NC = 3 if vec else 1
     E = np.zeros((nenv, NC))
     G = np.zeros((nenv, max(1, pk.ngslots), NC), dtype=complex)
-    for ti, (r0, r1, slot) in enumerate(pk.tasks):
+    for ti, (r0, r1, slot) in enumerate(pk.tasks[:, :3]):
         acc = np.zeros((nenv, NC), dtype=complex) + np.asarray(tc[ti]).reshape(1, -1)
         P1 = P2 = P3 = np.ones(nenv, dtype=complex)
         for r in range(r0, r1):
