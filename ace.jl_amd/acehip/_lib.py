@@ -37,6 +37,9 @@ class AcePack(C.Structure):
         ("nterms", C.c_int32), ("term_row", _i32p), ("term_scale", _f64p), ("term_conj", _i32p),
         ("ngslots", C.c_int32), ("gslot_ptr", _i32p), ("gslot_list", _i32p),
         ("nrec", C.c_int32), ("rec_hdr", C.POINTER(C.c_uint32)), ("rec_leaf", _i32p),
+        ("nfeat", C.c_int32), ("feat_rows", _i32p), ("feat_ptr", _i32p), ("feat_f", _i32p),
+        ("feat_coef", _f64p), ("fgrad_ptr", _i32p), ("fgrad_feat", _i32p), ("fgrad_f", _i32p),
+        ("fgrad_coef", _f64p),
         ("nwave_cfg", C.c_int32), ("wave_cfg_W", _i32p), ("wave_task_ptr", _i32p),
         ("wave_task_list", _i32p), ("wave_etask_ptr", _i32p), ("wave_etask_list", _i32p),
     ]
@@ -45,7 +48,8 @@ class AcePack(C.Structure):
 # every symbol declared in include/acehip.h
 EXPORTS = ("ace_ctx_create", "ace_set_params", "ace_set_tile", "ace_energy",
            "ace_energy_forces", "ace_basis_A", "ace_basis_AA", "ace_basis_B",
-           "ace_basis_dB", "ace_last_timing", "ace_set_validation", "ace_last_error",
+           "ace_basis_dB", "ace_surr_features", "ace_surr_energy_forces",
+           "ace_last_timing", "ace_set_validation", "ace_last_error",
            "ace_ctx_destroy", "ace_abi_version")
 
 _lib = None
@@ -73,6 +77,9 @@ def load(path: str | None = None):
     lib.ace_energy_forces.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
     for name in ("ace_basis_A", "ace_basis_AA", "ace_basis_B", "ace_basis_dB"):
         getattr(lib, name).argtypes = ev
+    lib.ace_surr_features.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.ace_surr_energy_forces.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_int, vp, vp, C.c_double,
+                                           C.c_double, vp, C.c_double, vp, vp, vp, vp, vp, vp]
     lib.ace_last_timing.argtypes = [vp, _f64p, _f64p, _f64p]
     lib.ace_set_validation.argtypes = [vp, C.c_int]
     lib.ace_last_error.restype = C.c_char_p
@@ -153,6 +160,15 @@ def make_pack(pk):
     s.nrec = len(pk.rec_hdr)
     s.rec_hdr = _ptr(arr(pk.rec_hdr, np.uint32), C.c_uint32)
     s.rec_leaf = _ptr(arr(pk.rec_leaf, np.int32), C.c_int32)
+    s.nfeat = len(pk.feat_rows)
+    s.feat_rows = _ptr(arr(pk.feat_rows, np.int32), C.c_int32)
+    s.feat_ptr = _ptr(arr(pk.feat_ptr, np.int32), C.c_int32)
+    s.feat_f = _ptr(arr(pk.feat_f, np.int32), C.c_int32)
+    s.feat_coef = _ptr(arr(pk.feat_coef, np.float64), C.c_double)
+    s.fgrad_ptr = _ptr(arr(pk.fgrad_ptr, np.int32), C.c_int32)
+    s.fgrad_feat = _ptr(arr(pk.fgrad_feat, np.int32), C.c_int32)
+    s.fgrad_f = _ptr(arr(pk.fgrad_f, np.int32), C.c_int32)
+    s.fgrad_coef = _ptr(arr(pk.fgrad_coef, np.float64), C.c_double)
     s.nwave_cfg = len(pk.wave_cfg_W)
     s.wave_cfg_W = _ptr(arr(pk.wave_cfg_W, np.int32), C.c_int32)
     s.wave_task_ptr = _ptr(arr(pk.wave_task_ptr, np.int32), C.c_int32)

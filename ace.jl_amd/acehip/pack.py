@@ -124,6 +124,77 @@ class Pack:
         self.orders = orders.astype(np.int32)
         self.maxorder = int(iAA2iA.shape[1])
         self.iAA2iA = iAA2iA.astype(np.int32)
+        if self.prop == 0:
+            self._build_features(rows_, cols_, np.asarray(vals_, dtype=np.float64), kk, cj, sgn)
+        else:
+            self._empty_features()
+
+    def _empty_features(self):
+        self.feat_rows = np.zeros(0, dtype=np.int32)
+        self.feat_ptr = np.zeros(1, dtype=np.int32)
+        self.feat_f = np.zeros((0, 2), dtype=np.int32)
+        self.feat_coef = np.zeros(0)
+        self.fgrad_ptr = np.zeros(len(self.pool) + 1, dtype=np.int32)
+        self.fgrad_feat = np.zeros(0, dtype=np.int32)
+        self.fgrad_f = np.zeros(0, dtype=np.int32)
+        self.fgrad_coef = np.zeros(0)
+
+    def _build_features(self, rows, cols, vals, kk, cj, sgn):
+        """Order <= 2 invariants (the surrogate's inputs): the B rows whose AA
+        columns all have order <= 2 (every B row has a single order).
+        Feature f = sum_terms coef * Re(A_f1 A_f2) with f1/f2 device factor ids
+        (-1: absent), coef = U * sign.  Gradient entries, grouped by target
+        pooled index k, give G[k] += w_f * coef * value(other) with
+        value(-1) = 1 (see programs.py for the definition of G)."""
+        o = self.orders[cols]
+        maxo = np.zeros(self.nB, dtype=np.int64)
+        np.maximum.at(maxo, rows, o)
+        feat_rows = np.nonzero(maxo <= 2)[0]
+        self.feat_rows = feat_rows.astype(np.int32)
+        fptr, ff, fc = [0], [], []
+        grad = [[] for _ in range(len(self.pool))]
+        dev = lambda a: int(kk[a] | (cj[a] * P.DEV_CONJ))  # noqa: E731
+        for fi, b in enumerate(feat_rows):
+            for z in np.nonzero(rows == b)[0]:
+                col = int(cols[z])
+                od = int(self.orders[col])
+                idx = self.iAA2iA[col, :od]
+                s = float(np.prod(sgn[idx])) if od else 1.0
+                c = float(vals[z]) * s
+                f = [dev(a) for a in idx] + [-1] * (2 - od)
+                ff.append(f)
+                fc.append(c)
+                # gradient entries: d/dA+[k] directly, d/dconj(A+[k]) conjugated
+                if od == 1:
+                    grad[f[0] & 0x7fff].append((fi, -1, c))
+                elif od == 2:
+                    for a_, b_ in ((0, 1), (1, 0)):
+                        fa, fb = f[a_], f[b_]
+                        k = fa & 0x7fff
+                        if fa & P.DEV_CONJ:
+                            # conj(d/dconj A+[k]) = conj(A_fb): flip fb's conjugation
+                            other = fb ^ P.DEV_CONJ if not self._is_m0(fb) else fb
+                        else:
+                            other = fb
+                        grad[k].append((fi, other, c))
+            fptr.append(len(ff))
+        self.feat_ptr = np.asarray(fptr, dtype=np.int32)
+        self.feat_f = np.asarray(ff, dtype=np.int32).reshape(-1, 2)
+        self.feat_coef = np.asarray(fc, dtype=np.float64)
+        gp, gf, go, gc = [0], [], [], []
+        for k in range(len(self.pool)):
+            for (fi, other, c) in grad[k]:
+                gf.append(fi)
+                go.append(other)
+                gc.append(c)
+            gp.append(len(gf))
+        self.fgrad_ptr = np.asarray(gp, dtype=np.int32)
+        self.fgrad_feat = np.asarray(gf, dtype=np.int32)
+        self.fgrad_f = np.asarray(go, dtype=np.int32)
+        self.fgrad_coef = np.asarray(gc, dtype=np.float64)
+
+    def _is_m0(self, f):
+        return self.pool[f & 0x7fff, 3] == 0
 
     def _build_records(self, tasks3):
         """64-byte records: 8 header dwords + REC_LEAVES coefficients.

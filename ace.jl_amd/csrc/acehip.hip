@@ -445,12 +445,21 @@ __device__ __forceinline__ double rec_coef(uint32_t lo, uint32_t hi) {
 __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
                                                const double2* __restrict__ Aplus,
                                                int64_t estride, double* __restrict__ E_out,
-                                               double2* __restrict__ Gbuf, double e0r) {
+                                               double2* __restrict__ Gbuf, double e0r,
+                                               const int32_t* __restrict__ env_list,
+                                               const int32_t* __restrict__ env_count) {
     extern __shared__ double2 As[];  // [npool][64]
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t env = (int64_t)blockIdx.x * 64 + lane;
-    const bool valid = env < nenv;
+    int64_t env = (int64_t)blockIdx.x * 64 + lane;
+    bool valid = env < nenv;
+    if (env_list) {
+        // environments listed by the caller (fallback of the learned surrogate)
+        const int64_t cnt = *env_count;
+        if ((int64_t)blockIdx.x * 64 >= cnt) return;  // whole workgroup idle: before any barrier
+        valid = env < cnt;
+        env = valid ? env_list[env] : 0;
+    }
     for (int k = wave; k < a.npool; k += a.W)
         As[k * 64 + lane] = valid ? Aplus[(int64_t)k * estride + env] : make_double2(0.0, 0.0);
     __syncthreads();
@@ -867,6 +876,119 @@ __global__ void k_nbr_env(int64_t nenv, const int32_t* __restrict__ off, int32_t
 }
 
 // ---------------------------------------------------------------------------
+// learned-surrogate support: order <= 2 invariants, their gradient, fallback
+
+struct FeatArgs {
+    int nfeat, npool;
+    const int32_t* feat_ptr;
+    const int32_t* feat_f;
+    const double* feat_coef;
+    const double* cfeat;
+    const int32_t* fgrad_ptr;
+    const int32_t* fgrad_feat;
+    const int32_t* fgrad_f;
+    const double* fgrad_coef;
+};
+
+__device__ __forceinline__ double2 aplus_val(const double2* Aplus, int64_t estride, int f, int64_t env) {
+    if (f < 0) return make_double2(1.0, 0.0);
+    double2 v = Aplus[(int64_t)(f & 0x7fff) * estride + env];
+    if (f & 0x8000) v.y = -v.y;
+    return v;
+}
+
+// lanes = environments.  X[f][nenv] (FP64), Xn[env][160] (BF16, normalized),
+// E2 = c0 + sum_f c_f X_f.
+__global__ __launch_bounds__(64) void k_feat(FeatArgs a, int64_t nenv, const double2* __restrict__ Aplus,
+                                             int64_t estride, const double* __restrict__ xmean,
+                                             const double* __restrict__ xinvstd, double* __restrict__ X,
+                                             __bf16* __restrict__ Xn, double* __restrict__ E2, double e0) {
+    const int64_t env = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (env >= nenv) return;
+    double e = e0;
+    for (int f0 = 0; f0 < 160; f0 += 8) {
+        __bf16 xb[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int f = f0 + q;
+            double v = 0.0;
+            if (f < a.nfeat) {
+                for (int z = a.feat_ptr[f]; z < a.feat_ptr[f + 1]; ++z) {
+                    const double2 a1 = aplus_val(Aplus, estride, a.feat_f[2 * z], env);
+                    const double2 a2 = aplus_val(Aplus, estride, a.feat_f[2 * z + 1], env);
+                    v += a.feat_coef[z] * (a1.x * a2.x - a1.y * a2.y);
+                }
+                e += a.cfeat[f] * v;
+                if (X) X[(int64_t)f * nenv + env] = v;
+                xb[q] = (__bf16)(float)((v - xmean[f]) * xinvstd[f]);
+            } else {
+                xb[q] = (__bf16)0.0f;
+            }
+        }
+        if (Xn) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Xn[env * 160 + f0 + q] = xb[q];
+        }
+    }
+    E2[env] = e;
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// E = E2 + ymean + ystd * mean_m y_m; fallback where the spread exceeds tau
+__global__ void k_surr_combine(int64_t nenv, int M, const float* __restrict__ y, double ymean, double ystd,
+                               double tau, const double* __restrict__ E2, double* __restrict__ E_out,
+                               int32_t* __restrict__ fb_count, int32_t* __restrict__ fb_list) {
+    const int64_t env = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (env >= nenv) return;
+    double s = 0.0, s2 = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const double v = y[(int64_t)m * nenv + env];
+        s += v;
+        s2 += v * v;
+    }
+    const double mean = s / M;
+    const double var = fmax(s2 / M - mean * mean, 0.0);
+    E_out[env] = E2[env] + ymean + ystd * mean;
+    if (ystd * sqrt(var) > tau) {
+        const int pos = atomicAdd(fb_count, 1);
+        fb_list[pos] = (int32_t)env;
+    }
+}
+
+// feature weights w_f = c_f + ystd * xinvstd_f * mean_m dX[m][env][f], then
+// G[k] = sum_entries w_f coef value(other)   (lanes = environments)
+__global__ __launch_bounds__(64) void k_feat_grad(FeatArgs a, int64_t nenv, const double2* __restrict__ Aplus,
+                                                  int64_t estride, int M, const float* __restrict__ dX,
+                                                  double ystd, const double* __restrict__ xinvstd,
+                                                  double* __restrict__ Wf, const int32_t* __restrict__ gslot_ptr,
+                                                  const int32_t* __restrict__ gslot_list,
+                                                  double2* __restrict__ Gbuf) {
+    const int64_t env = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (env >= nenv) return;
+    for (int f = 0; f < a.nfeat; ++f) {
+        double s = 0.0;
+        for (int m = 0; m < M; ++m) s += dX[((int64_t)m * nenv + env) * 160 + f];
+        Wf[(int64_t)f * estride + env] = a.cfeat[f] + ystd * xinvstd[f] * s / M;
+    }
+    for (int k = 0; k < a.npool; ++k) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int z = a.fgrad_ptr[k]; z < a.fgrad_ptr[k + 1]; ++z) {
+            const double w = Wf[(int64_t)a.fgrad_feat[z] * estride + env] * a.fgrad_coef[z];
+            const double2 v = aplus_val(Aplus, estride, a.fgrad_f[z], env);
+            acc.x += w * v.x;
+            acc.y += w * v.y;
+        }
+        const int s0 = gslot_ptr[k], s1 = gslot_ptr[k + 1];
+        for (int q = s0; q < s1; ++q)
+            Gbuf[(int64_t)gslot_list[q] * estride + env] = q == s0 ? acc : make_double2(0.0, 0.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 template <typename T>
@@ -916,6 +1038,12 @@ struct ace_ctx {
     DevBuf<int32_t> runs, leaf_fid, d_tasks, gslot_ptr, gslot_list;
     DevBuf<double> leaf_coef, task_const, e0v;
     DevBuf<uint32_t> recs;
+    // surrogate feature tables
+    int nfeat = 0;
+    std::vector<int32_t> feat_rows;
+    DevBuf<int32_t> feat_ptr, feat_f, fgrad_ptr, fgrad_feat, fgrad_f;
+    DevBuf<double> feat_coef, fgrad_coef, cfeat, Wf;
+    int64_t surr_nenv = -1;
     double e0r = 0.0;
     std::vector<WaveCfg*> wcfg;
     int cur_w = 0;
@@ -943,6 +1071,11 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
     if (ctx->task_const.upload(cf.task.data(), cf.task.size())) return -1;
     ctx->e0r = cf.e0[0];
     double ev[3] = {cf.e0[0], cf.e0[2], cf.e0[4]};
+    if (ctx->nfeat > 0) {
+        std::vector<double> cf_(ctx->nfeat);
+        for (int f = 0; f < ctx->nfeat; ++f) cf_[f] = c[ctx->feat_rows[f]];
+        if (ctx->cfeat.upload(cf_.data(), cf_.size())) return -1;
+    }
     return ctx->e0v.upload(ev, 3);
 }
 
@@ -1051,6 +1184,16 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             wc->d_list_all.upload(wc->list_all.data(), wc->list_all.size()) ||
             wc->d_ptr_e.upload(wc->ptr_e.data(), wc->ptr_e.size()) ||
             wc->d_list_e.upload(wc->list_e.data(), wc->list_e.size()))
+            return fail("upload");
+    }
+    ctx->nfeat = pk->nfeat;
+    if (pk->nfeat > 0) {
+        const int nt = pk->feat_ptr[pk->nfeat], ng = pk->fgrad_ptr[pk->npool];
+        ctx->feat_rows.assign(pk->feat_rows, pk->feat_rows + pk->nfeat);
+        if (ctx->feat_ptr.upload(pk->feat_ptr, pk->nfeat + 1) || ctx->feat_f.upload(pk->feat_f, 2 * (size_t)nt) ||
+            ctx->feat_coef.upload(pk->feat_coef, nt) || ctx->fgrad_ptr.upload(pk->fgrad_ptr, pk->npool + 1) ||
+            ctx->fgrad_feat.upload(pk->fgrad_feat, ng) || ctx->fgrad_f.upload(pk->fgrad_f, ng) ||
+            ctx->fgrad_coef.upload(pk->fgrad_coef, ng))
             return fail("upload");
     }
     if (upload_coeffs(ctx, c)) return fail("set_params");
@@ -1231,7 +1374,8 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
         ra.W = wc->W;
         set_func_lds((const void*)k_dagr, lds);
         hipLaunchKernelGGL(k_dagr, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, ra, nenv,
-                           ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r);
+                           ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr);
     } else if ((size_t)ctx->npool * 64 * sizeof(double2) <= 160 * 1024) {
         set_func_lds((const void*)k_dag<1, 64>, lds);
         hipLaunchKernelGGL((k_dag<1, 64>), dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
@@ -1402,5 +1546,96 @@ extern "C" int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
         hipLaunchKernelGGL(k_dB<1>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, nnbr, ctx->nB,
                            ctx->nAA, ctx->rowptr.p, ctx->cols.p, ctx->vals.p, ctx->dAA.p, dB_out);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ---- learned surrogate
+
+static FeatArgs feat_args(ace_ctx* ctx) {
+    FeatArgs a;
+    a.nfeat = ctx->nfeat;
+    a.npool = ctx->npool;
+    a.feat_ptr = ctx->feat_ptr.p;
+    a.feat_f = ctx->feat_f.p;
+    a.feat_coef = ctx->feat_coef.p;
+    a.cfeat = ctx->cfeat.p;
+    a.fgrad_ptr = ctx->fgrad_ptr.p;
+    a.fgrad_feat = ctx->fgrad_feat.p;
+    a.fgrad_f = ctx->fgrad_f.p;
+    a.fgrad_coef = ctx->fgrad_coef.p;
+    return a;
+}
+
+extern "C" int ace_surr_features(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                                 const int8_t* species, const double* xmean, const double* xinvstd, double* X,
+                                 void* Xn, double* E2, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (ctx->prop != ACE_PROP_INVARIANT || ctx->nfeat < 1) return set_err("surrogate features need an Invariant basis");
+    if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
+    if (nenv == 0) return 0;
+    if (!xmean || !xinvstd || !E2) return set_err("ace_surr_features: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t estride = ((nenv + 63) / 64) * 64;
+    if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
+    DISPATCH_L(ctx->L, launch_pool, ctx, st, nenv, off, xyz, nnbr, species, estride);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_feat, dim3((unsigned)((nenv + 63) / 64)), dim3(64), 0, st, feat_args(ctx), nenv,
+                       ctx->Aplus.p, estride, xmean, xinvstd, X, (__bf16*)Xn, E2, ctx->e0r);
+    HIP_TRY(hipGetLastError());
+    ctx->surr_nenv = nenv;
+    return 0;
+}
+
+extern "C" int ace_surr_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                                      const int8_t* species, int nmemb, const float* y, const float* dX,
+                                      double ymean, double ystd, const double* xinvstd, double tau,
+                                      const double* E2, double* E_out, double* dE, int32_t* fb_count,
+                                      int32_t* fb_list, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (ctx->prop != ACE_PROP_INVARIANT || ctx->nfeat < 1) return set_err("surrogate needs an Invariant basis");
+    if (nenv == 0) return 0;
+    if (ctx->surr_nenv != nenv) return set_err("ace_surr_energy_forces must follow ace_surr_features on the same batch");
+    if (nmemb < 1 || nmemb > 16 || !y || !E2 || !E_out || !fb_count || !fb_list || !xinvstd)
+        return set_err("ace_surr_energy_forces: bad arguments");
+    if (dE && !dX) return set_err("forces need the input gradients dX");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t ntile = (nenv + 63) / 64;
+    const int64_t estride = ntile * 64;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, fb_count, 1);
+    hipLaunchKernelGGL(k_surr_combine, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, nmemb, y, ymean,
+                       ystd, tau, E2, E_out, fb_count, fb_list);
+    HIP_TRY(hipGetLastError());
+    const bool forces = dE != nullptr;
+    if (forces) {
+        if (ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * estride) ||
+            ctx->Wf.ensure((size_t)ctx->nfeat * estride))
+            return -1;
+        hipLaunchKernelGGL(k_feat_grad, dim3((unsigned)ntile), dim3(64), 0, st, feat_args(ctx), nenv, ctx->Aplus.p,
+                           estride, nmemb, dX, ystd, xinvstd, ctx->Wf.p, ctx->gslot_ptr.p, ctx->gslot_list.p,
+                           ctx->Gbuf.p);
+        HIP_TRY(hipGetLastError());
+    }
+    // plain ACE for the listed environments (energy, and G for forces)
+    WaveCfg* wc = ctx->wcfg[ctx->cur_w];
+    DagRecArgs ra;
+    ra.recs = ctx->recs.p;
+    ra.tasks = ctx->d_tasks.p;
+    ra.task_const = ctx->task_const.p;
+    ra.wtask_ptr = forces ? wc->d_ptr_all.p : wc->d_ptr_e.p;
+    ra.wtask_list = forces ? wc->d_list_all.p : wc->d_list_e.p;
+    ra.npool = ctx->npool;
+    ra.W = wc->W;
+    size_t lds = std::max((size_t)ctx->npool * 64 * sizeof(double2), (size_t)wc->W * 64 * sizeof(double));
+    if (lds > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
+    set_func_lds((const void*)k_dagr, lds);
+    hipLaunchKernelGGL(k_dagr, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, ra, nenv, ctx->Aplus.p, estride,
+                       E_out, ctx->Gbuf.p, ctx->e0r, (const int32_t*)fb_list, (const int32_t*)fb_count);
+    HIP_TRY(hipGetLastError());
+    if (forces && nnbr > 0) {
+        DISPATCH_L(ctx->L, launch_force_any, ctx, st, nenv, off, xyz, nnbr, species, estride, dE);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
 }

@@ -104,6 +104,29 @@ inline std::string validate_pack(const ace_pack* pk) {
     if (pk->gslot_ptr[pk->npool] > pk->ngslots) return "bad gslot table";
     for (int s = 0; s < pk->gslot_ptr[pk->npool]; ++s)
         if (pk->gslot_list[s] < 0 || pk->gslot_list[s] >= pk->ngslots) return "gslot out of range";
+    if (pk->nfeat < 0 || pk->nfeat > 160) return "nfeat must be in [0, 160]";
+    if (pk->nfeat > 0) {
+        if (!pk->feat_rows || !pk->feat_ptr || !pk->feat_f || !pk->feat_coef || !pk->fgrad_ptr)
+            return "null feature table";
+        if (pk->feat_ptr[0] != 0) return "bad feature table";
+        for (int f = 0; f < pk->nfeat; ++f) {
+            if (pk->feat_rows[f] < 0 || pk->feat_rows[f] >= pk->nB) return "feature row out of range";
+            if (pk->feat_ptr[f + 1] < pk->feat_ptr[f]) return "bad feature table";
+        }
+        const int nt = pk->feat_ptr[pk->nfeat];
+        for (int z = 0; z < 2 * nt; ++z)
+            if (pk->feat_f[z] < -1 || (pk->feat_f[z] >= 0 && (pk->feat_f[z] & 0x7fff) >= pk->npool))
+                return "feature factor out of range";
+        if (pk->fgrad_ptr[0] != 0) return "bad feature gradient table";
+        for (int k = 0; k < pk->npool; ++k)
+            if (pk->fgrad_ptr[k + 1] < pk->fgrad_ptr[k]) return "bad feature gradient table";
+        const int ng = pk->fgrad_ptr[pk->npool];
+        for (int z = 0; z < ng; ++z) {
+            if (pk->fgrad_feat[z] < 0 || pk->fgrad_feat[z] >= pk->nfeat) return "feature gradient entry out of range";
+            if (pk->fgrad_f[z] < -1 || (pk->fgrad_f[z] >= 0 && (pk->fgrad_f[z] & 0x7fff) >= pk->npool))
+                return "feature gradient entry out of range";
+        }
+    }
     if (pk->nwave_cfg < 1) return "no wave configuration";
     return "";
 }

@@ -108,6 +108,20 @@ typedef struct ace_pack {
     int32_t nrec;
     const uint32_t* rec_hdr;      /* [nrec][8] */
     const int32_t* rec_leaf;      /* [nrec][4] leaf index per slot, -1 empty */
+    /* order <= 2 invariants (surrogate inputs): feature f = sum over terms
+     * z in [feat_ptr[f], feat_ptr[f+1]) of feat_coef[z] * Re(A_f1 A_f2) with
+     * (f1, f2) = feat_f[z][0..1] device factor ids (-1 = absent); feat_rows[f]
+     * is its B row.  Gradient entries per pooled index k: fgrad_ptr[k]..,
+     * (feature, other factor, coef): G[k] += w_feature coef value(other). */
+    int32_t nfeat;
+    const int32_t* feat_rows;     /* [nfeat] */
+    const int32_t* feat_ptr;      /* [nfeat+1] */
+    const int32_t* feat_f;        /* [nterm][2] */
+    const double* feat_coef;      /* [nterm] */
+    const int32_t* fgrad_ptr;     /* [npool+1] */
+    const int32_t* fgrad_feat;    /* [ngrad] */
+    const int32_t* fgrad_f;       /* [ngrad] */
+    const double* fgrad_coef;     /* [ngrad] */
     /* task -> wave assignment for each supported waves-per-tile setting */
     int32_t nwave_cfg;
     const int32_t* wave_cfg_W;    /* [nwave_cfg] waves per workgroup */
@@ -166,6 +180,34 @@ int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
 int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
                  const double* xyz, const int8_t* species, double* dB_out,
                  void* stream);
+
+/* Learned-surrogate support (no reference counterpart; for the surrogate
+ * path of a LinearACEModel, see acemlp.h).
+ *
+ * ace_surr_features: pooled 1p values (kept in the context for the next
+ * ace_surr_energy_forces call), the order <= 2 invariants X[nfeat][nenv]
+ * (FP64, nullable), their normalized BF16 copy Xn[nenv][160] =
+ * (X - xmean) * xinvstd zero-padded (nullable), and the exact order <= 2
+ * energy share E2[nenv] = c0 + sum_f c_f X_f.  xmean / xinvstd are device
+ * arrays [nfeat]. */
+int ace_surr_features(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                      const int8_t* species, const double* xmean, const double* xinvstd, double* X,
+                      void* Xn, double* E2, void* stream);
+
+/* Surrogate energies (and forces): with member predictions y[nmemb][nenv]
+ * of the normalized order-3/4 share and (for forces) their input gradients
+ * dX[nmemb][nenv][160], E = E2 + ymean + ystd * mean_m y_m.  Environments
+ * whose ensemble spread ystd * std_m y_m exceeds tau fall back to plain ACE
+ * (the product DAG on this context); their indices are written to
+ * fallback_list and their count to *fallback_count (device int32).  Forces
+ * of accepted environments follow from the chain rule through the features;
+ * dE_dr as in ace_energy_forces.  Must follow ace_surr_features on the same
+ * batch and context. */
+int ace_surr_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                           const int8_t* species, int nmemb, const float* y, const float* dX,
+                           double ymean, double ystd, const double* xinvstd, double tau,
+                           const double* E2, double* E_out, double* dE_dr, int32_t* fallback_count,
+                           int32_t* fallback_list, void* stream);
 
 /* Kernel-time instrumentation: average duration in ms of the last call's
  * product-DAG kernel (measured with HIP events on the call's stream) and the
