@@ -56,7 +56,7 @@ def _vp(t):
 class MLPEnsemble:
     """nmemb MLPs (nin -> 128 -> 128 -> 128 -> 1, SiLU) on the device."""
 
-    def __init__(self, nmemb=4, nin=152, device=0, seed=0):
+    def __init__(self, nmemb=4, nin=151, device=0, seed=0):
         self.lib = load()
         self.nmemb, self.nin, self.device = nmemb, nin, device
         self.nparam = self.lib.acemlp_nparam()

@@ -131,6 +131,7 @@ class Pack:
 
     def _empty_features(self):
         self.feat_rows = np.zeros(0, dtype=np.int32)
+        self.const_rows = np.zeros(0, dtype=np.int32)
         self.feat_ptr = np.zeros(1, dtype=np.int32)
         self.feat_f = np.zeros((0, 2), dtype=np.int32)
         self.feat_coef = np.zeros(0)
@@ -140,8 +141,10 @@ class Pack:
         self.fgrad_coef = np.zeros(0)
 
     def _build_features(self, rows, cols, vals, kk, cj, sgn):
-        """Order <= 2 invariants (the surrogate's inputs): the B rows whose AA
-        columns all have order <= 2 (every B row has a single order).
+        """Order 1 and 2 invariants (the surrogate's inputs): the B rows whose AA
+        columns all have order 1 or 2 (every B row has a single order).  The
+        order-0 row is the constant already carried by the energy constant,
+        so it is not a feature.
         Feature f = sum_terms coef * Re(A_f1 A_f2) with f1/f2 device factor ids
         (-1: absent), coef = U * sign.  Gradient entries, grouped by target
         pooled index k, give G[k] += w_f * coef * value(other) with
@@ -149,8 +152,9 @@ class Pack:
         o = self.orders[cols]
         maxo = np.zeros(self.nB, dtype=np.int64)
         np.maximum.at(maxo, rows, o)
-        feat_rows = np.nonzero(maxo <= 2)[0]
+        feat_rows = np.nonzero((maxo >= 1) & (maxo <= 2))[0]
         self.feat_rows = feat_rows.astype(np.int32)
+        self.const_rows = np.nonzero(maxo == 0)[0].astype(np.int32)
         fptr, ff, fc = [0], [], []
         grad = [[] for _ in range(len(self.pool))]
         dev = lambda a: int(kk[a] | (cj[a] * P.DEV_CONJ))  # noqa: E731

@@ -23,6 +23,13 @@ sys.path.insert(0, os.path.join(ROOT, "ace.jl_amd"))
 METRIC = "env/s energy+forces: plain FP64 vs surrogate at ≤1 meV/atom MAE"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per k_dagr launch on this workload (100k environments, 16-wave
+# tiles), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script,
+# scaled by the factors a 256 MiB device copy gives the same counters (2.0 for
+# FETCH_SIZE, 1.0 for WRITE_SIZE): 570.9 MB read + 224.8 MB written.  Only
+# valid for the default --nenv; re-measured whenever the kernel changes.
+DAG_TRAFFIC_BYTES = {100_000: 795.7e6}
+TRAFFIC_SOURCE = "profiles/plain_ef_config2_kernels.md"
 
 
 def algorithmic_flops(pack, forces=True):
@@ -124,7 +131,7 @@ def surrogate_leg(ev, b, E_plain, steps, warmup, ntrain, neval, epochs, log):
             "eval_mae_per_atom": tot_err / tot_n, "eval_fallback_fraction": tot_fb / tot_n, "eval_envs": tot_n,
             "tau": tau, "calib_fallback_fraction": frac_cal, "calib_mae": mae_cal,
             "energy_unit": "model units (eV assumed); target MAE 1e-3 (1 meV/atom)",
-            "ensemble": "4 x MLP 152->128->128->128->1 SiLU, BF16 MFMA / FP32 accumulate",
+            "ensemble": f"{s.nmemb} x MLP {s.nfeat}->128->128->128->1 SiLU, BF16 MFMA / FP32 accumulate",
             "train": {"envs": info["rows"], "epochs": epochs, "steps": info["steps"],
                       "seconds": round(info["seconds"], 1), "final_loss": info["loss"][-1]}}
 
@@ -225,8 +232,10 @@ def main():
                    "autotune_ms": None if tuned is None else {str(k): round(v, 3) for k, v in tuned[1].items()}},
         "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (no MFMA); MI355X FP64 peak",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_dag<0>", "kernel_ms": round(dag_avg, 4),
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": DAG_TRAFFIC_BYTES.get(b.nenv),
+                     "traffic_source": TRAFFIC_SOURCE if b.nenv in DAG_TRAFFIC_BYTES else None,
+                     "kernel": "k_dagr", "kernel_ms": round(dag_avg, 4),
                      "flops_per_env_algorithmic": f_alg, "flops_per_env_executed": f_exe,
                      "executed_tflops": round(achieved_exe, 3),
                      "executed_frac": round(achieved_exe / FP64_PEAK_TFLOPS, 4),

@@ -25,7 +25,7 @@ def _batch(nenv, seed):
 
 
 def test_features_and_E2(setup):
-    print("[gpu] surrogate features and exact order<=2 energy", flush=True)
+    print("[gpu] surrogate features and exact order<=2 energy share", flush=True)
     from acehip.surrogate import Surrogate
     basis, T, c, ev = setup
     s = Surrogate(ev, nmemb=2, seed=0)
@@ -36,7 +36,8 @@ def test_features_and_E2(setup):
     rows = ev.pack.feat_rows
     assert rel_err(X.cpu().numpy().T, B[:, rows]) < 1e-12
     c2 = np.zeros_like(c)
-    c2[rows] = c[rows]
+    keep = np.concatenate([rows, ev.pack.const_rows])   # E2 carries the constant term
+    c2[keep] = c[keep]
     assert rel_err(E2.cpu().numpy(), O.energy(T, c2, off, R)) < 1e-12
     # BF16 normalized copy, zero padding
     assert torch.all(Xn[:, len(rows):] == 0)
@@ -66,7 +67,8 @@ def test_chain_rule_forces(setup):
         ce[rows] = w
         ref = O.grad_config(T, ce, off[e:e + 2] - off[e], R[off[e]:off[e + 1]])
         assert rel_err(g[off[e]:off[e + 1]], ref) < 1e-10
-    E2 = O.energy(T, np.where(np.isin(np.arange(len(c)), rows), c, 0.0), off, R)
+    keep = np.concatenate([rows, ev.pack.const_rows])
+    E2 = O.energy(T, np.where(np.isin(np.arange(len(c)), keep), c, 0.0), off, R)
     Eml = E2 + s.ymean + s.ystd * y.double().mean(0).cpu().numpy()
     assert rel_err(E.cpu().numpy(), Eml) < 1e-12
 
