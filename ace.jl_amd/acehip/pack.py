@@ -31,9 +31,10 @@ class Pack:
         iAA2iA = np.asarray(T["iAA2iA"], dtype=np.int64)
         if iAA2iA.shape[1] > 4:
             raise ValueError("correlation order above 4 is not supported")
-        poly, _ = P.energy_polynomial(orders, iAA2iA, kk, cj, sgn)
-        self.poly = poly
         is_m0 = self.pool[:, 3] == 0
+        poly, _ = P.energy_polynomial(orders, iAA2iA, kk, cj, sgn)
+        poly = P.merge_conjugates(poly, is_m0)
+        self.poly = poly
         G = P.gradient_polynomials(poly, len(self.pool), is_m0)
         prog = P.RunProgram()
         self.energy_tasks = [prog.add_task(part, -1) for part in P.split_poly(poly, ENERGY_PARTS)]

@@ -74,6 +74,36 @@ def energy_polynomial(orders, iAA2iA, kk, cj, sgn):
     return poly, const_terms
 
 
+def merge_conjugates(poly, is_m0):
+    """Merge every monomial with its mirror image.
+
+    The property is ``Re P`` (Invariant: ``E = Re sum_AA c~ AA``; the
+    EuclideanVector components likewise), and ``Re(c x) = Re(conj(c)
+    conj(x))``.  A monomial and its conjugate (every factor ``A+[k]`` swapped
+    with ``conj(A+[k])``; m = 0 pooled values are real and map to themselves)
+    therefore contribute through one monomial whose coefficient is the sum of
+    the first coefficient and the conjugate of the second.  The mirror pairs
+    are the AA terms of m and -m, related by Y_l^{-m} = (-1)^m conj(Y_l^m)
+    (sphericalharmonics.jl:394-398): the config-2 energy shrinks from 35.7k
+    to 20.2k monomials.  ``Re P`` is unchanged, and so is the adjoint G[k] of
+    every m != 0 pooled value (it is fixed by dE = Re sum_k G[k] dA+[k] with
+    complex dA+[k]); for m = 0 pooled values only Re G[k] is fixed, and only
+    Re G[k] enters the forces."""
+
+    def conj_id(x):
+        k = x >> 1
+        return 2 * k if is_m0[k] else (x ^ 1)
+
+    out = {}
+    for mono, terms in poly.items():
+        c = tuple(sorted(conj_id(f) for f in mono))
+        if c < mono:
+            out.setdefault(c, []).extend((row, s, 1 - cj) for (row, s, cj) in terms)
+        else:
+            out.setdefault(mono, []).extend(terms)
+    return out
+
+
 def gradient_polynomials(poly, npool, is_m0):
     """``G[k]`` polynomials (list over k) with symbolic coefficients.
 

@@ -1,9 +1,9 @@
 """Learned surrogate for the correlation-order-3/4 share of the site energy.
 
 No reference counterpart (ACE.jl is a linear model).  Inputs are the
-order <= 2 invariants of the same SymmetricBasis (the B functions whose AA
-columns have order <= 2); the exact order <= 2 energy share
-``E2 = c0 + sum_f c_f X_f`` is kept in FP64.  An ensemble of MLPs
+order-1 and order-2 invariants of the same SymmetricBasis (the B functions
+whose AA columns have order 1 or 2); the exact order <= 2 energy share
+``E2 = c0 + sum_f c_f X_f`` (c0 the order-0 constant) is kept in FP64.  An ensemble of MLPs
 (libacemlp.so, BF16 MFMA, FP32 accumulate) predicts the normalized remainder
 ``E34 = E - E2``; plain FP64 ACE (libacehip.so) produces the training labels
 and is the fallback wherever the ensemble spread exceeds ``tau``.
@@ -28,7 +28,7 @@ from .mlp import INP, MLPEnsemble
 class Surrogate:
     def __init__(self, ev: HIPEvaluator, nmemb: int = 4, seed: int = 0):
         if ev.prop != 0 or len(ev.pack.feat_rows) == 0:
-            raise ValueError("the surrogate needs an Invariant basis with order <= 2 functions")
+            raise ValueError("the surrogate needs an Invariant basis with order-1 or order-2 functions")
         self.ev = ev
         self.nfeat = len(ev.pack.feat_rows)
         self.nmemb = nmemb

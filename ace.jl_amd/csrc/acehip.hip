@@ -194,7 +194,13 @@ __device__ __forceinline__ int64_t xcd_env(int64_t b, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// k_pool: A+[k][env]
+// k_pool: A+[k][env].  One wave per environment.  Neighbours are staged
+// POOL_CH at a time in LDS ([slot][R_n | Y_lm | species]); the lanes then
+// turn into pooled functions k = lane + 64 p (p < POOL_KP, npool <= 256)
+// and sum over the staged neighbours in order, holding the sums in registers
+// across chunks.
+constexpr int POOL_CH = 32;
+constexpr int POOL_KP = 4;
 
 template <int L>
 __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
@@ -204,21 +210,34 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
                                               const int32_t* __restrict__ pool, int npool,
                                               double2* __restrict__ Aplus, int64_t estride) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
-    extern __shared__ double smem[];
+    extern __shared__ double smem[];  // [POOL_CH][rstride]
     const int lane = threadIdx.x;
     const int64_t env = xcd_env(blockIdx.x, nenv);
     const int nr = rp.nr;
     const int rstride = nr + 2 * NP + 1;  // doubles per neighbour slot
-    double* base = smem;
     if (env >= nenv) return;
     const int j0 = off[env], j1 = off[env + 1];
-
-    for (int jb = j0; jb < j1 || jb == j0; jb += 64) {
-        const int nch = max(0, min(64, j1 - jb));
+    // this lane's pooled functions k = lane + 64 p: slot offsets of R_n and Y_lm
+    int kr[POOL_KP], ky[POOL_KP], kmu[POOL_KP];
+    double ar[POOL_KP], ai[POOL_KP];
+#pragma unroll
+    for (int p = 0; p < POOL_KP; ++p) {
+        const int k = lane + 64 * p;
+        kr[p] = ky[p] = 0;
+        kmu[p] = -2;  // -2: no pooled function in this pass
+        if (k < npool) {
+            kmu[p] = pool[4 * k];
+            kr[p] = pool[4 * k + 1] - 1;
+            ky[p] = nr + 2 * idxp(pool[4 * k + 2], pool[4 * k + 3]);
+        }
+        ar[p] = ai[p] = 0.0;
+    }
+    for (int jb = j0; jb < j1; jb += POOL_CH) {
+        const int nch = min(POOL_CH, j1 - jb);
         if (lane < nch) {
             const int j = jb + lane;
             const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
-            double* slot = base + lane * rstride;
+            double* slot = smem + lane * rstride;
             Sph S = cart2spher(x, y, z);
             // radial basis
             double u = (1.0 + rp.r0) / (1.0 + S.r);
@@ -264,30 +283,25 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // lanes over pooled functions; partial sums of earlier neighbour
-        // chunks are read back from HBM (same lane wrote them)
-        for (int k = lane; k < npool; k += 64) {
-            const int mu = pool[4 * k], n = pool[4 * k + 1] - 1, l = pool[4 * k + 2],
-                      m = pool[4 * k + 3];
-            const int iy = idxp(l, m);
-            double ar = 0.0, ai = 0.0;
-            if (jb != j0) {
-                const double2 prev = Aplus[(int64_t)k * estride + env];
-                ar = prev.x;
-                ai = prev.y;
+        // lanes over pooled functions, neighbours summed in order
+        for (int jj = 0; jj < nch; ++jj) {
+            const double* slot = smem + jj * rstride;
+            const int smu = species ? (int)slot[nr + 2 * NP] : -1;
+#pragma unroll
+            for (int p = 0; p < POOL_KP; ++p) {
+                if (kmu[p] == -1 || kmu[p] == smu) {
+                    const double R = slot[kr[p]];
+                    ar[p] += R * slot[ky[p]];
+                    ai[p] += R * slot[ky[p] + 1];
+                }
             }
-            for (int jj = 0; jj < nch; ++jj) {
-                const double* slot = base + jj * rstride;
-                if (mu >= 0 && (int)slot[nr + 2 * NP] != mu) continue;
-                const double R = slot[n];
-                ar += R * slot[nr + 2 * iy];
-                ai += R * slot[nr + 2 * iy + 1];
-            }
-            Aplus[(int64_t)k * estride + env] = make_double2(ar, ai);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
+#pragma unroll
+    for (int p = 0; p < POOL_KP; ++p)
+        if (kmu[p] != -2) Aplus[(int64_t)(lane + 64 * p) * estride + env] = make_double2(ar[p], ai[p]);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,11 +544,12 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
     }
 }
 
+
 // ---------------------------------------------------------------------------
 // k_force: +dE/dr_j = Re sum_k G[k] grad(R_n Y_lm)(r_j)
 
 template <int L, int PROP>
-__global__ __launch_bounds__(64) void k_force(RadialParams rp, int64_t nenv,
+__global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
                                                const int32_t* __restrict__ off,
                                                const double* __restrict__ xyz, int64_t nnbr,
                                                const int8_t* __restrict__ species,
@@ -961,30 +976,51 @@ __global__ void k_surr_combine(int64_t nenv, int M, const float* __restrict__ y,
 
 // feature weights w_f = c_f + ystd * xinvstd_f * mean_m dX[m][env][f], then
 // G[k] = sum_entries w_f coef value(other)   (lanes = environments)
-__global__ __launch_bounds__(64) void k_feat_grad(FeatArgs a, int64_t nenv, const double2* __restrict__ Aplus,
-                                                  int64_t estride, int M, const float* __restrict__ dX,
-                                                  double ystd, const double* __restrict__ xinvstd,
-                                                  double* __restrict__ Wf, const int32_t* __restrict__ gslot_ptr,
-                                                  const int32_t* __restrict__ gslot_list,
-                                                  double2* __restrict__ Gbuf) {
-    const int64_t env = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (env >= nenv) return;
-    for (int f = 0; f < a.nfeat; ++f) {
-        double s = 0.0;
-        for (int m = 0; m < M; ++m) s += dX[((int64_t)m * nenv + env) * 160 + f];
-        Wf[(int64_t)f * estride + env] = a.cfeat[f] + ystd * xinvstd[f] * s / M;
+// Adjoint of the accepted environments:
+//   w_f = c_f + ystd * invstd_f * mean_m dy_m/dX_f
+//   G[k] = sum_z w_{feat(z)} coef_z value(other_z)
+// One workgroup per 64 environments.  dX [M][nenv][160] is read coalesced
+// (consecutive threads = consecutive features of one environment) and the
+// weights are kept in LDS as [f][65] doubles (padded: the transposed writes
+// fall in different banks).  Waves then split the pooled index k; lanes are
+// environments and read A+ coalesced from [k][env].
+constexpr int FG_PAD = 65;
+__global__ __launch_bounds__(256) void k_feat_grad(FeatArgs a, int64_t nenv, const double2* __restrict__ Aplus,
+                                                   int64_t estride, int M, const float* __restrict__ dX,
+                                                   double ystd, const double* __restrict__ xinvstd,
+                                                   const int32_t* __restrict__ gslot_ptr,
+                                                   const int32_t* __restrict__ gslot_list,
+                                                   double2* __restrict__ Gbuf) {
+    extern __shared__ double Wl[];  // [nfeat][FG_PAD]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t env0 = (int64_t)blockIdx.x * 64;
+    const int nloc = (int)std::min<int64_t>(64, nenv - env0);
+    const double invM = 1.0 / M;
+    for (int idx = tid; idx < 64 * 160; idx += 256) {
+        const int e = idx / 160, f = idx - e * 160;
+        if (f < a.nfeat) {
+            double sm = 0.0;
+            if (e < nloc)
+                for (int m = 0; m < M; ++m) sm += (double)dX[((int64_t)m * nenv + env0 + e) * 160 + f];
+            Wl[f * FG_PAD + e] = a.cfeat[f] + ystd * xinvstd[f] * sm * invM;
+        }
     }
-    for (int k = 0; k < a.npool; ++k) {
+    __syncthreads();
+    const int64_t env = env0 + lane;
+    const bool valid = lane < nloc;
+    for (int k = wave; k < a.npool; k += 4) {
         double2 acc = make_double2(0.0, 0.0);
         for (int z = a.fgrad_ptr[k]; z < a.fgrad_ptr[k + 1]; ++z) {
-            const double w = Wf[(int64_t)a.fgrad_feat[z] * estride + env] * a.fgrad_coef[z];
-            const double2 v = aplus_val(Aplus, estride, a.fgrad_f[z], env);
+            const double w = Wl[a.fgrad_feat[z] * FG_PAD + lane] * a.fgrad_coef[z];
+            const double2 v = valid ? aplus_val(Aplus, estride, a.fgrad_f[z], env) : make_double2(0.0, 0.0);
             acc.x += w * v.x;
             acc.y += w * v.y;
         }
-        const int s0 = gslot_ptr[k], s1 = gslot_ptr[k + 1];
-        for (int q = s0; q < s1; ++q)
-            Gbuf[(int64_t)gslot_list[q] * estride + env] = q == s0 ? acc : make_double2(0.0, 0.0);
+        if (valid) {
+            const int s0 = gslot_ptr[k], s1 = gslot_ptr[k + 1];
+            for (int q = s0; q < s1; ++q)
+                Gbuf[(int64_t)gslot_list[q] * estride + env] = q == s0 ? acc : make_double2(0.0, 0.0);
+        }
     }
 }
 
@@ -1035,6 +1071,7 @@ struct ace_ctx {
     DevBuf<int32_t> pool, aspec_fid, aspec4, orders, iAA2iA, rowptr, cols;
     DevBuf<double> aspec_sgn, vals;
     DevBuf<int16_t> kidx;
+    int nrl[ACE_LMAX + 1] = {0};  // n bound per l over the pooled functions
     DevBuf<int32_t> runs, leaf_fid, d_tasks, gslot_ptr, gslot_list;
     DevBuf<double> leaf_coef, task_const, e0v;
     DevBuf<uint32_t> recs;
@@ -1042,7 +1079,7 @@ struct ace_ctx {
     int nfeat = 0;
     std::vector<int32_t> feat_rows;
     DevBuf<int32_t> feat_ptr, feat_f, fgrad_ptr, fgrad_feat, fgrad_f;
-    DevBuf<double> feat_coef, fgrad_coef, cfeat, Wf;
+    DevBuf<double> feat_coef, fgrad_coef, cfeat;
     int64_t surr_nenv = -1;
     double e0r = 0.0;
     std::vector<WaveCfg*> wcfg;
@@ -1078,6 +1115,7 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
     }
     return ctx->e0v.upload(ev, 3);
 }
+
 
 extern "C" int ace_abi_version(void) { return ACEHIP_ABI_VERSION; }
 
@@ -1171,6 +1209,8 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         if (ctx->aspec4.upload(a4.data(), a4.size())) return fail("upload");
         std::vector<int16_t> kx = acehost::pool_lookup(pk);
         if (ctx->kidx.upload(kx.data(), kx.size())) return fail("upload");
+        for (int k = 0; k < pk->npool; ++k)
+            ctx->nrl[pk->pool[4 * k + 2]] = std::max(ctx->nrl[pk->pool[4 * k + 2]], (int)pk->pool[4 * k + 1]);
     }
     for (auto& t : wt) {
         WaveCfg* wc = new WaveCfg();
@@ -1288,11 +1328,196 @@ template <int L>
 static void launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
                         const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
-    const size_t lds = (size_t)64 * (ctx->nr + 2 * NP + 1) * sizeof(double);
+    const size_t lds = (size_t)POOL_CH * (ctx->nr + 2 * NP + 1) * sizeof(double);
     set_func_lds((const void*)k_pool<L>, lds);
     hipLaunchKernelGGL(k_pool<L>, dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx), nenv,
                        off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->pool.p,
                        ctx->npool, ctx->Aplus.p, estride);
+}
+
+// k_force: one wave per environment, lanes are neighbours.  The
+// environment's adjoint is staged in LDS as a dense table
+// Gd[mu][l, m >= 0][n] (zero where the pooled function is absent), so the
+// contraction needs no index lookup.  R_n and dR_n stay in registers (nr <=
+// 16); the ALP values are produced m-outer, l-inner, by the same operations
+// as alp_eval_ed, so only the two previous degrees are live.
+constexpr int FORCE_NRMAX = 16;
+struct ForceTab {
+    int8_t nrl[ACE_LMAX + 1];  // n bound per l (max pooled n over mu, m)
+};
+
+template <int L, int PROP>
+__global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int64_t nenv,
+                                               const int32_t* __restrict__ off,
+                                               const double* __restrict__ xyz, int64_t nnbr,
+                                               const int8_t* __restrict__ species, int nsp,
+                                               const int32_t* __restrict__ pool, int npool,
+                                               const int32_t* __restrict__ gslot_ptr,
+                                               const int32_t* __restrict__ gslot_list,
+                                               const double2* __restrict__ Gbuf, int64_t estride,
+                                               double* __restrict__ dE) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    extern __shared__ double2 Gd[];  // [nsp][NP][nr][NC]
+    const int lane = threadIdx.x;
+    const int64_t env = xcd_env(blockIdx.x, nenv);
+    const int nr = rp.nr;
+    if (env >= nenv) return;
+    const int ntab = nsp * NP * nr * NC;
+    for (int q = lane; q < ntab; q += 64) Gd[q] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int k = lane; k < npool; k += 64) {
+        const int mu = max(0, pool[4 * k]), n = pool[4 * k + 1] - 1, l = pool[4 * k + 2], m = pool[4 * k + 3];
+        const int d = ((mu * NP + idxp(l, m)) * nr + n) * NC;
+#pragma unroll
+        for (int p = 0; p < NC; ++p) {
+            double2 g = make_double2(0.0, 0.0);
+            for (int s = gslot_ptr[k]; s < gslot_ptr[k + 1]; ++s) {
+                const double2 v = Gbuf[((int64_t)gslot_list[s] * NC + p) * estride + env];
+                g.x += v.x;
+                g.y += v.y;
+            }
+            Gd[d + p] = g;
+        }
+    }
+    __syncthreads();
+    const int j0 = off[env], j1 = off[env + 1];
+    for (int j = j0 + lane; j < j1; j += 64) {
+        const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+        const int mu = species ? (int)species[j] : 0;
+        const double2* Gm = Gd + (int64_t)mu * NP * nr * NC;
+        const Sph S = cart2spher(x, y, z);
+        double t, dtdr;
+        transform_ed(rp, S.r, t, dtdr);
+        double R[FORCE_NRMAX], dR[FORCE_NRMAX];
+        {
+            double p0 = rp.A[0] * fcut(rp, t), d0 = rp.A[0] * fcut_d(rp, t);
+            R[0] = p0;
+            dR[0] = d0;
+            double p1 = 0.0, d1 = 0.0;
+            if (nr > 1) {
+                const double al = rp.A[1] * t + rp.B[1];
+                p1 = al * p0;
+                d1 = al * d0 + rp.A[1] * p0;
+            }
+            R[1] = p1;
+            dR[1] = d1;
+#pragma unroll
+            for (int n = 2; n < FORCE_NRMAX; ++n) {
+                if (n < nr) {
+                    const double al = rp.A[n] * t + rp.B[n];
+                    const double p2 = al * p1 + rp.C[n] * p0;
+                    const double d2 = al * d1 + rp.C[n] * d0 + rp.A[n] * p1;
+                    p0 = p1;
+                    p1 = p2;
+                    d0 = d1;
+                    d1 = d2;
+                }
+                R[n] = p1;
+                dR[n] = d1;
+            }
+        }
+        const double rinv = 1.0 / (S.r + 2.220446049250313e-16);
+        const double rx = x / S.r, ry = y / S.r, rz = z / S.r;
+        const double st2 = S.sth * S.sth;
+        double g[NC][3];
+#pragma unroll
+        for (int p = 0; p < NC; ++p) g[p][0] = g[p][1] = g[p][2] = 0.0;
+        double epr = 0.7071067811865475, epi = 0.0;
+        double Pmm = 0.3989422804014327, dPmm = 0.0;  // P_{m,m}, dP_{m,m} (P/sin(theta) for m >= 1)
+        const double temp0 = 0.3989422804014327;
+#pragma unroll
+        for (int m = 0; m <= L; ++m) {
+            if (m == 1) {
+                Pmm = -1.224744871391589 * temp0;
+                dPmm = -1.224744871391589 * (S.cth * temp0);
+            } else if (m >= 2) {
+                const double f = sqrt(1.0 + 0.5 / m);
+                const double nt1 = -f * S.sth * Pmm;
+                const double ntd = -f * (S.cth * Pmm * S.sth + S.sth * dPmm);
+                Pmm = nt1;
+                dPmm = ntd;
+            }
+            if (m > 0) {
+                const double nr_ = epr * S.cphi - epi * S.sphi;
+                const double ni_ = epr * S.sphi + epi * S.cphi;
+                epr = nr_;
+                epi = ni_;
+            }
+            double Pa = 0.0, dPa = 0.0, Pb = Pmm, dPb = dPmm;  // degrees l-2, l-1 of order m
+#pragma unroll
+            for (int l = m; l <= L; ++l) {
+                double Pl, dPl;
+                if (l == m) {
+                    Pl = Pmm;
+                    dPl = dPmm;
+                } else if (l == m + 1) {
+                    if (m == 0) {
+                        Pl = S.cth * 1.7320508075688772 * temp0;
+                        dPl = -S.sth * 1.7320508075688772 * temp0;
+                    } else {
+                        const double c = sqrt(2.0 * m + 3.0);
+                        Pl = c * S.cth * Pmm;
+                        dPl = c * (-st2 * Pmm + S.cth * dPmm);
+                    }
+                } else {
+                    const int i = idxp(l, m);
+                    Pl = c_alpA[i] * (S.cth * Pb + c_alpB[i] * Pa);
+                    dPl = m == 0 ? c_alpA[i] * (-S.sth * Pb + S.cth * dPb + c_alpB[i] * dPa)
+                                 : c_alpA[i] * (-st2 * Pb + S.cth * dPb + c_alpB[i] * dPa);
+                }
+                if (l > m) {
+                    Pa = Pb;
+                    dPa = dPb;
+                    Pb = Pl;
+                    dPb = dPl;
+                }
+                // Y and its Cartesian gradient (cYlm_ed!, dspher_to_dcart)
+                double Yr, Yi, fpr, fpi, ftr, fti;
+                if (m == 0) {
+                    Yr = Pl * 0.7071067811865475;
+                    Yi = 0.0;
+                    fpr = fpi = 0.0;
+                    ftr = dPl * 0.7071067811865475;
+                    fti = 0.0;
+                } else {
+                    Yr = epr * Pl * S.sth;
+                    Yi = epi * Pl * S.sth;
+                    fpr = -m * epi * Pl;
+                    fpi = m * epr * Pl;
+                    ftr = epr * dPl;
+                    fti = epi * dPl;
+                }
+                const double dYr[3] = {(-S.sphi * fpr + S.cphi * S.cth * ftr) * rinv,
+                                       (S.cphi * fpr + S.sphi * S.cth * ftr) * rinv,
+                                       (-S.sth * ftr) * rinv};
+                const double dYi[3] = {(-S.sphi * fpi + S.cphi * S.cth * fti) * rinv,
+                                       (S.cphi * fpi + S.sphi * S.cth * fti) * rinv,
+                                       (-S.sth * fti) * rinv};
+                const double2* Gl = Gm + idxp(l, m) * nr * NC;
+                const int nb = ft.nrl[l];
+#pragma unroll
+                for (int n = 0; n < FORCE_NRMAX; ++n) {
+                    if (n < nb) {
+                        const double Rn = R[n], dRn = dR[n] * dtdr;
+#pragma unroll
+                        for (int p = 0; p < NC; ++p) {
+                            const double2 G = Gl[n * NC + p];
+                            // Re(G * (dR rhat Y + R dY))
+                            const double gy = G.x * Yr - G.y * Yi;
+                            g[p][0] += dRn * gy * rx + Rn * (G.x * dYr[0] - G.y * dYi[0]);
+                            g[p][1] += dRn * gy * ry + Rn * (G.x * dYr[1] - G.y * dYi[1]);
+                            g[p][2] += dRn * gy * rz + Rn * (G.x * dYr[2] - G.y * dYi[2]);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < NC; ++p)
+#pragma unroll
+            for (int dd = 0; dd < 3; ++dd) dE[(int64_t)(p * 3 + dd) * nnbr + j] = g[p][dd];
+    }
 }
 
 template <int L, int PROP>
@@ -1300,9 +1525,21 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
                          const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride,
                          double* dE) {
     constexpr int NC = PROP == 0 ? 1 : 3;
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    const int nsp = std::max(1, ctx->nspecies);
+    const size_t dense = (size_t)nsp * NP * ctx->nr * NC * sizeof(double2);
+    if (ctx->nr <= FORCE_NRMAX && dense <= 64 * 1024) {
+        ForceTab ft;
+        for (int l = 0; l <= ACE_LMAX; ++l) ft.nrl[l] = (int8_t)(l <= L ? ctx->nrl[l] : 0);
+        set_func_lds((const void*)k_force<L, PROP>, dense);
+        hipLaunchKernelGGL((k_force<L, PROP>), dim3((unsigned)nenv), dim3(64), dense, st, radial_params(ctx), ft,
+                           nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, nsp, ctx->pool.p,
+                           ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ctx->Gbuf.p, estride, dE);
+        return;
+    }
     const size_t lds = (size_t)ctx->npool * NC * sizeof(double2) + 64 * 2 * ctx->nr * sizeof(double);
-    set_func_lds((const void*)k_force<L, PROP>, lds);
-    hipLaunchKernelGGL((k_force<L, PROP>), dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx),
+    set_func_lds((const void*)k_force_lds<L, PROP>, lds);
+    hipLaunchKernelGGL((k_force_lds<L, PROP>), dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx),
                        nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->kidx.p,
                        ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ctx->Gbuf.p, estride, dE);
 }
@@ -1609,11 +1846,11 @@ extern "C" int ace_surr_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t*
     HIP_TRY(hipGetLastError());
     const bool forces = dE != nullptr;
     if (forces) {
-        if (ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * estride) ||
-            ctx->Wf.ensure((size_t)ctx->nfeat * estride))
-            return -1;
-        hipLaunchKernelGGL(k_feat_grad, dim3((unsigned)ntile), dim3(64), 0, st, feat_args(ctx), nenv, ctx->Aplus.p,
-                           estride, nmemb, dX, ystd, xinvstd, ctx->Wf.p, ctx->gslot_ptr.p, ctx->gslot_list.p,
+        if (ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * estride)) return -1;
+        const size_t fg_lds = (size_t)ctx->nfeat * FG_PAD * sizeof(double);
+        set_func_lds((const void*)k_feat_grad, fg_lds);
+        hipLaunchKernelGGL(k_feat_grad, dim3((unsigned)ntile), dim3(256), fg_lds, st, feat_args(ctx), nenv,
+                           ctx->Aplus.p, estride, nmemb, dX, ystd, xinvstd, ctx->gslot_ptr.p, ctx->gslot_list.p,
                            ctx->Gbuf.p);
         HIP_TRY(hipGetLastError());
     }

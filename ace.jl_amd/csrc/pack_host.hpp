@@ -27,7 +27,7 @@ inline std::string validate_pack(const ace_pack* pk) {
     if (pk->prop != ACE_PROP_INVARIANT && pk->prop != ACE_PROP_EUCLIDEAN_VECTOR) return "unknown property";
     if (pk->L < 0 || pk->L > 8) return "L must be in [0, 8]";
     if (pk->nr < 1 || pk->nr > 32) return "nr must be in [1, 32]";
-    if (pk->npool < 1 || pk->npool > 0x7fff) return "npool out of range";
+    if (pk->npool < 1 || pk->npool > 256) return "npool out of range (the pooling kernel holds at most 256 pooled functions)";
     if (pk->nspecies < 0 || pk->nspecies > 127) return "nspecies out of range";
     if (pk->tp < 1 || pk->tp > 16) return "transform power out of range";
     if (pk->pl < 0 || pk->pr < 0 || pk->pl > 16 || pk->pr > 16) return "envelope powers out of range";
@@ -173,6 +173,7 @@ struct PackCopy {
     int prop = 0, nAA = 0, nB = 0, nleaves = 0, ntasks = 0, nrec = 0;
     std::vector<uint32_t> rec_hdr;
     std::vector<int32_t> rec_leaf;
+    std::vector<int32_t> tasks;  // [ntasks][5] (run_b, run_e, slot, rec_b, rec_e)
     std::vector<int32_t> rowptr, cols, leaf_term_ptr, task_term_ptr, term_row, term_conj, orders;
     std::vector<double> vals, term_scale;
     int ncoef() const { return prop == ACE_PROP_INVARIANT ? 1 : 6; }
@@ -198,6 +199,7 @@ inline PackCopy copy_pack(const ace_pack* pk) {
     p.nrec = pk->nrec;
     p.rec_hdr.assign(pk->rec_hdr, pk->rec_hdr + 8 * (size_t)pk->nrec);
     p.rec_leaf.assign(pk->rec_leaf, pk->rec_leaf + 4 * (size_t)pk->nrec);
+    p.tasks.assign(pk->tasks, pk->tasks + 5 * (size_t)pk->ntasks);
     return p;
 }
 

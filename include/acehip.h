@@ -185,10 +185,10 @@ int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
  * path of a LinearACEModel, see acemlp.h).
  *
  * ace_surr_features: pooled 1p values (kept in the context for the next
- * ace_surr_energy_forces call), the order <= 2 invariants X[nfeat][nenv]
+ * ace_surr_energy_forces call), the order-1 and order-2 invariants X[nfeat][nenv]
  * (FP64, nullable), their normalized BF16 copy Xn[nenv][160] =
  * (X - xmean) * xinvstd zero-padded (nullable), and the exact order <= 2
- * energy share E2[nenv] = c0 + sum_f c_f X_f.  xmean / xinvstd are device
+ * energy share E2[nenv] = c0 + sum_f c_f X_f (c0: the order-0 constant).  xmean / xinvstd are device
  * arrays [nfeat]. */
 int ace_surr_features(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
                       const int8_t* species, const double* xmean, const double* xinvstd, double* X,
