@@ -427,9 +427,10 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
 }
 
 // ---------------------------------------------------------------------------
-// k_dagr: the same programs as 64-byte records (Invariant).  A record holds
-// the run header and up to four leaves with their coefficients; the next
-// record is loaded while the current record's LDS reads are in flight.
+// k_dagr: the same programs as 64-byte decoded records (Invariant).  A
+// record holds one run (or the continuation of a long run): up to four leaves
+// with their coefficients and the prefix levels it recomputes, all as ready
+// LDS byte offsets, so the loop spends few scalar instructions per record.
 
 struct DagRecArgs {
     const uint32_t* recs;      // [nrec + 1][16]
@@ -440,6 +441,17 @@ struct DagRecArgs {
     int npool;
     int W;
 };
+
+// Streaming accesses: the A+ tile is read once and G written once per
+// environment; the non-temporal hint keeps them from evicting the program
+// records, which every tile re-reads, from L2.
+__device__ __forceinline__ double2 nt_load2(const double2* p) {
+    return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+__device__ __forceinline__ void nt_store2(double2* p, double2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
 
 __device__ __forceinline__ double flip_sign_if(double c, uint32_t f) {
     // negate c when the conjugation bit (0x8000) of factor id f is set
@@ -454,6 +466,50 @@ __device__ __forceinline__ double2 lds_conj(const double2* As, uint32_t f, int l
 
 __device__ __forceinline__ double rec_coef(uint32_t lo, uint32_t hi) {
     return __hiloint2double((int)hi, (int)lo);
+}
+
+// One record of a given type (MORE: leaves 2 and 3 present; U1 / U2 / U3:
+// prefix levels recomputed): its LDS reads first, then its arithmetic.
+template <bool MORE, bool U1, bool U2, bool U3>
+__device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const uint4& h0, const uint4& h1,
+                                            const uint4& h2, const uint4& h3, double2& acc, double2& P1,
+                                            double2& P2, double2& Pc) {
+    auto at = [&](uint32_t off) { return *reinterpret_cast<const double2*>(lds + off); };
+    const double2 v0 = at(h0.y), v1 = at(h0.z);
+    double2 v2, v3, a1, a2, a3;
+    if (MORE) {
+        v2 = at(h0.w);
+        v3 = at(h1.x);
+    }
+    if (U1) a1 = at(h1.y & 0x7fffffffu);
+    if (U2) a2 = at(h1.z & 0x7fffffffu);
+    if (U3) a3 = at(h1.w & 0x7fffffffu);
+    const double c0 = rec_coef(h2.x, h2.y), c1 = rec_coef(h2.z, h2.w);
+    double2 S;
+    S.x = c0 * v0.x + c1 * v1.x;
+    S.y = c0 * v0.y + c1 * v1.y;
+    if (MORE) {
+        const double c2 = rec_coef(h3.x, h3.y), c3 = rec_coef(h3.z, h3.w);
+        S.x += c2 * v2.x + c3 * v3.x;
+        S.y += c2 * v2.y + c3 * v3.y;
+    }
+    S.y = flip_sign_if(S.y, ctl >> 16);  // conjugated leaves: conj(sum c A) = sum c conj(A)
+    if (U1) {
+        a1.y = flip_sign_if(a1.y, h1.y >> 16);
+        P1 = a1;
+        Pc = a1;
+    }
+    if (U2) {
+        a2.y = flip_sign_if(a2.y, h1.z >> 16);
+        P2 = cmul(P1, a2);
+        Pc = P2;
+    }
+    if (U3) {
+        a3.y = flip_sign_if(a3.y, h1.w >> 16);
+        Pc = cmul(P2, a3);
+    }
+    acc.x += Pc.x * S.x - Pc.y * S.y;
+    acc.y += Pc.x * S.y + Pc.y * S.x;
 }
 
 __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
@@ -475,54 +531,38 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
         env = valid ? env_list[env] : 0;
     }
     for (int k = wave; k < a.npool; k += a.W)
-        As[k * 64 + lane] = valid ? Aplus[(int64_t)k * estride + env] : make_double2(0.0, 0.0);
+        As[k * 64 + lane] = valid ? nt_load2(Aplus + (int64_t)k * estride + env) : make_double2(0.0, 0.0);
     __syncthreads();
 
+    const char* lds = reinterpret_cast<const char*>(As) + 16 * lane;
     double ep = 0.0;
     const int tb = a.wtask_ptr[wave], te = a.wtask_ptr[wave + 1];
     for (int ti = tb; ti < te; ++ti) {
         const int t = a.wtask_list[ti];
         const int slot = a.tasks[5 * t + 2], rb = a.tasks[5 * t + 3], re = a.tasks[5 * t + 4];
         double2 acc = make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
-        double2 P1 = make_double2(1.0, 0.0), P2 = P1, P3 = P1;
+        // P1, P2: prefix products of levels 1 and 2; Pc: the product of the
+        // current record's whole prefix.  Runs are sorted, so every new run
+        // recomputes its deepest level and Pc is always the last level computed
+        // (1 for the prefix-free runs, which come first).
+        double2 P1 = make_double2(1.0, 0.0), P2 = P1, Pc = P1;
         const uint4* rp = reinterpret_cast<const uint4*>(a.recs) + 4 * (int64_t)rb;
         uint4 h0 = rp[0], h1 = rp[1], h2 = rp[2], h3 = rp[3];
         for (int r = rb; r < re; ++r) {
             // next record (record nrec exists and is all zeros)
             const uint4* np = reinterpret_cast<const uint4*>(a.recs) + 4 * (int64_t)(r + 1);
             const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-            const uint32_t hd = h0.x;
-            const int d = hd & 3, j = (hd >> 2) & 7, nl = (hd >> 5) & 7;
-            const uint32_t l0 = h0.w & 0xffffu, l1 = h0.w >> 16;
-            // leaf values (slots 0, 1 always; 2, 3 when present)
-            const double2 v0 = As[(l0 & 0x7fffu) * 64 + lane];
-            const double2 v1 = As[(l1 & 0x7fffu) * 64 + lane];
-            const double c0 = rec_coef(h2.x, h2.y), c1 = rec_coef(h2.z, h2.w);
-            double2 S;
-            S.x = c0 * v0.x + c1 * v1.x;
-            S.y = flip_sign_if(c0, l0) * v0.y + flip_sign_if(c1, l1) * v1.y;
-            if (nl > 2) {
-                const uint32_t l2 = h1.x & 0xffffu, l3 = h1.x >> 16;
-                const double2 v2 = As[(l2 & 0x7fffu) * 64 + lane];
-                const double2 v3 = As[(l3 & 0x7fffu) * 64 + lane];
-                const double c2 = rec_coef(h3.x, h3.y), c3 = rec_coef(h3.z, h3.w);
-                S.x += c2 * v2.x + c3 * v3.x;
-                S.y += flip_sign_if(c2, l2) * v2.y + flip_sign_if(c3, l3) * v3.y;
+            const uint32_t ctl = h0.x;
+#define ACE_REC(T)                                                                                         \
+    case T:                                                                                                \
+        dagr_record<(T & 1) != 0, (T & 2) != 0, (T & 4) != 0, (T & 8) != 0>(lds, ctl, h0, h1, h2, h3, acc, \
+                                                                              P1, P2, Pc);                 \
+        break;
+            switch ((ctl >> 2) & 15) {
+                ACE_REC(0) ACE_REC(1) ACE_REC(2) ACE_REC(3) ACE_REC(4) ACE_REC(5) ACE_REC(6) ACE_REC(7)
+                ACE_REC(8) ACE_REC(9) ACE_REC(10) ACE_REC(11) ACE_REC(12) ACE_REC(13) ACE_REC(14) ACE_REC(15)
             }
-            // prefix products for the levels that changed
-            if (j <= d) {
-                if (j <= 1) P1 = lds_conj(As, h0.y & 0xffffu, lane);
-                if (j <= 2 && d >= 2) P2 = cmul(P1, lds_conj(As, h0.y >> 16, lane));
-                if (d >= 3) P3 = cmul(P2, lds_conj(As, h0.z & 0xffffu, lane));
-            }
-            if (d == 0) {
-                acc.x += S.x;
-                acc.y += S.y;
-            } else {
-                const double2 Pd = d == 1 ? P1 : (d == 2 ? P2 : P3);
-                acc.x += Pd.x * S.x - Pd.y * S.y;
-                acc.y += Pd.x * S.y + Pd.y * S.x;
-            }
+#undef ACE_REC
             h0 = n0;
             h1 = n1;
             h2 = n2;
@@ -531,7 +571,7 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
         if (slot < 0)
             ep += acc.x;
         else if (valid)
-            Gbuf[(int64_t)slot * estride + env] = acc;
+            nt_store2(Gbuf + (int64_t)slot * estride + env, acc);
     }
     __syncthreads();
     double* red = reinterpret_cast<double*>(As);
@@ -543,6 +583,7 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
         E_out[env] = s;
     }
 }
+
 
 
 // ---------------------------------------------------------------------------
@@ -1102,7 +1143,9 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
     acehost::Coeffs cf = acehost::coefficients(ctx->host, c);
     if (ctx->leaf_coef.upload(cf.leaf.data(), cf.leaf.size())) return -1;
     if (ctx->prop == ACE_PROP_INVARIANT) {
-        std::vector<uint32_t> rec = acehost::records(ctx->host, cf);
+        std::vector<uint32_t> rec;
+        const std::string rerr = acehost::records(ctx->host, cf, rec);
+        if (!rerr.empty()) return set_err(rerr);
         if (ctx->recs.upload(rec.data(), rec.size())) return -1;
     }
     if (ctx->task_const.upload(cf.task.data(), cf.task.size())) return -1;
@@ -1115,6 +1158,7 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
     }
     return ctx->e0v.upload(ev, 3);
 }
+
 
 
 extern "C" int ace_abi_version(void) { return ACEHIP_ABI_VERSION; }

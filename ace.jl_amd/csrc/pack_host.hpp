@@ -237,13 +237,40 @@ inline Coeffs coefficients(const PackCopy& p, const double* c) {
     return out;
 }
 
-// Device records (Invariant): 8 header dwords + 4 coefficients, plus one
-// zero record at the end so the kernel can always prefetch record r + 1.
-inline std::vector<uint32_t> records(const PackCopy& p, const Coeffs& cf) {
-    std::vector<uint32_t> out(16 * ((size_t)p.nrec + 1), 0u);
+// Device records (Invariant), decoded for the kernel: 64 bytes each.
+//   dw0    control: REC_MORE (leaves 2 and 3 present), REC_U1 / REC_U2 /
+//          REC_U3 (prefix level 1 / 2 / 3 is recomputed) -- together the
+//          record type in bits 2-5 --, REC_CONJ (the leaves are conjugated:
+//          one flag per record, since the leaf's m is fixed by the prefix)
+//   dw1-4  leaf i: LDS byte offset of its pooled value (k << 10; 64 lanes x 16 B)
+//   dw5-7  prefix factor t: LDS byte offset | conjugation in bit 31
+//   dw8-15 leaf coefficients (FP64; zero for absent leaves)
+// One zero record follows the last so the kernel can always load record r + 1.
+constexpr uint32_t REC_MORE = 1u << 2, REC_U1 = 1u << 3, REC_U2 = 1u << 4, REC_U3 = 1u << 5;
+constexpr uint32_t REC_CONJ = 1u << 31;
+
+// Returns "" or a message (a record whose leaves mix conjugation).
+inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint32_t>& out) {
+    out.assign(16 * ((size_t)p.nrec + 1), 0u);
+    auto off = [](uint32_t f) { return (f & 0x7fffu) << 10; };
     for (int r = 0; r < p.nrec; ++r) {
+        const uint32_t* h = p.rec_hdr.data() + 8 * (size_t)r;
         uint32_t* o = out.data() + 16 * (size_t)r;
-        for (int q = 0; q < 8; ++q) o[q] = p.rec_hdr[8 * (size_t)r + q];
+        const int d = h[0] & 3, j = (h[0] >> 2) & 7, nl = (h[0] >> 5) & 7;
+        uint32_t ctl = 0;
+        if (nl > 2) ctl |= REC_MORE;
+        if (j <= d && j <= 1) ctl |= REC_U1;
+        if (j <= d && j <= 2 && d >= 2) ctl |= REC_U2;
+        if (j <= d && d >= 3) ctl |= REC_U3;
+        const uint32_t lid[4] = {h[3] & 0xffffu, h[3] >> 16, h[4] & 0xffffu, h[4] >> 16};
+        const uint32_t fid[3] = {h[1] & 0xffffu, h[1] >> 16, h[2] & 0xffffu};
+        const bool cj = (lid[0] & 0x8000u) != 0;
+        for (int i = 0; i < nl; ++i)
+            if (((lid[i] & 0x8000u) != 0) != cj) return "record leaves mix conjugation";
+        if (cj) ctl |= REC_CONJ;
+        o[0] = ctl;
+        for (int i = 0; i < 4; ++i) o[1 + i] = i < nl ? off(lid[i]) : 0u;
+        for (int t = 0; t < 3; ++t) o[5 + t] = off(fid[t]) | ((fid[t] & 0x8000u) << 16);
         double c[4] = {0, 0, 0, 0};
         for (int i = 0; i < 4; ++i) {
             const int lf = p.rec_leaf[4 * (size_t)r + i];
@@ -251,7 +278,7 @@ inline std::vector<uint32_t> records(const PackCopy& p, const Coeffs& cf) {
         }
         std::memcpy(o + 8, c, sizeof(c));
     }
-    return out;
+    return "";
 }
 
 // (mu, l, m, n) -> pooled index, -1 when absent (force kernel lookup).
