@@ -208,7 +208,8 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
                                               const double* __restrict__ xyz, int64_t nnbr,
                                               const int8_t* __restrict__ species,
                                               const int32_t* __restrict__ pool, int npool,
-                                              double2* __restrict__ Aplus, int64_t estride) {
+                                              double2* __restrict__ Aplus, int64_t estride,
+                                              int32_t* __restrict__ nbr_env) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
     extern __shared__ double smem[];  // [POOL_CH][rstride]
     const int lane = threadIdx.x;
@@ -236,6 +237,7 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
         const int nch = min(POOL_CH, j1 - jb);
         if (lane < nch) {
             const int j = jb + lane;
+            if (nbr_env) nbr_env[j] = (int32_t)env;  // neighbour -> environment, for the flat force kernel
             const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
             double* slot = smem + lane * rstride;
             Sph S = cart2spher(x, y, z);
@@ -1099,6 +1101,12 @@ struct WaveCfg {
     DevBuf<int32_t> d_ptr_all, d_list_all, d_ptr_e, d_list_e;
 };
 
+constexpr int FORCE_NRMAX = 16;  // radial functions held in registers by the force kernels
+struct ForceFlatTab {
+    int8_t nrl[ACE_LMAX + 1];                                  // n bound per l
+    int16_t slot[(ACE_LMAX + 1) * (ACE_LMAX + 2) / 2][FORCE_NRMAX];  // G slot of (l, m >= 0, n)
+};
+
 struct ace_ctx {
     int device = 0;
     int prop = 0, L = 0, nr = 0, nspecies = 0, tp = 2, pl = 0, pr = 0;
@@ -1113,6 +1121,12 @@ struct ace_ctx {
     DevBuf<double> aspec_sgn, vals;
     DevBuf<int16_t> kidx;
     int nrl[ACE_LMAX + 1] = {0};  // n bound per l over the pooled functions
+    // flat force kernel (Invariant, one species, nr <= 16): slot table and the
+    // split gradient slots folded into their primary slot before it runs
+    bool flat_force = false;
+    ForceFlatTab fflat;
+    int nfold = 0;
+    DevBuf<int32_t> gfold;
     DevBuf<int32_t> runs, leaf_fid, d_tasks, gslot_ptr, gslot_list;
     DevBuf<double> leaf_coef, task_const, e0v;
     DevBuf<uint32_t> recs;
@@ -1255,6 +1269,39 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         if (ctx->kidx.upload(kx.data(), kx.size())) return fail("upload");
         for (int k = 0; k < pk->npool; ++k)
             ctx->nrl[pk->pool[4 * k + 2]] = std::max(ctx->nrl[pk->pool[4 * k + 2]], (int)pk->pool[4 * k + 1]);
+        ctx->flat_force = pk->prop == ACE_PROP_INVARIANT && pk->nspecies <= 1 && pk->nr <= FORCE_NRMAX &&
+                          pk->ngslots > 0;
+        if (ctx->flat_force) {
+            std::memset(&ctx->fflat, 0, sizeof(ctx->fflat));
+            for (int l = 0; l <= ACE_LMAX; ++l) ctx->fflat.nrl[l] = (int8_t)ctx->nrl[l];
+            // every (l, m, n) inside the n bounds must be a pooled function (true
+            // for the prefix-shaped pooled sets of RnYlm bases); otherwise the
+            // per-environment kernel runs
+            std::vector<int> seen((size_t)(ACE_LMAX + 1) * (ACE_LMAX + 2) / 2 * FORCE_NRMAX, 0);
+            std::vector<int32_t> fold;
+            for (int k = 0; k < pk->npool; ++k) {
+                const int n = pk->pool[4 * k + 1] - 1, l = pk->pool[4 * k + 2], m = pk->pool[4 * k + 3];
+                const int s0 = pk->gslot_ptr[k], s1 = pk->gslot_ptr[k + 1];
+                if (s1 <= s0) {
+                    ctx->flat_force = false;  // a pooled function without a gradient slot
+                    break;
+                }
+                ctx->fflat.slot[idxp(l, m)][n] = (int16_t)pk->gslot_list[s0];
+                seen[(size_t)idxp(l, m) * FORCE_NRMAX + n] = 1;
+                for (int q = s0 + 1; q < s1; ++q) {
+                    fold.push_back(pk->gslot_list[s0]);
+                    fold.push_back(pk->gslot_list[q]);
+                }
+            }
+            for (int l = 0; l <= pk->L && ctx->flat_force; ++l)
+                for (int m = 0; m <= l; ++m)
+                    for (int n = 0; n < ctx->nrl[l]; ++n)
+                        if (!seen[(size_t)idxp(l, m) * FORCE_NRMAX + n]) ctx->flat_force = false;
+            if (pk->ngslots > 32767) ctx->flat_force = false;
+            ctx->nfold = (int)fold.size() / 2;
+            if (ctx->flat_force && ctx->nfold > 0 && ctx->gfold.upload(fold.data(), fold.size()))
+                return fail("upload");
+        }
     }
     for (auto& t : wt) {
         WaveCfg* wc = new WaveCfg();
@@ -1374,9 +1421,14 @@ static void launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_
     constexpr int NP = (L + 1) * (L + 2) / 2;
     const size_t lds = (size_t)POOL_CH * (ctx->nr + 2 * NP + 1) * sizeof(double);
     set_func_lds((const void*)k_pool<L>, lds);
+    int32_t* nbr_env = nullptr;
+    if (ctx->flat_force && nnbr > 0) {
+        if (ctx->nbr_env.ensure(nnbr)) return;  // error recorded; the force launch checks it again
+        nbr_env = ctx->nbr_env.p;
+    }
     hipLaunchKernelGGL(k_pool<L>, dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx), nenv,
                        off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->pool.p,
-                       ctx->npool, ctx->Aplus.p, estride);
+                       ctx->npool, ctx->Aplus.p, estride, nbr_env);
 }
 
 // k_force: one wave per environment, lanes are neighbours.  The
@@ -1385,7 +1437,6 @@ static void launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_
 // contraction needs no index lookup.  R_n and dR_n stay in registers (nr <=
 // 16); the ALP values are produced m-outer, l-inner, by the same operations
 // as alp_eval_ed, so only the two previous degrees are live.
-constexpr int FORCE_NRMAX = 16;
 struct ForceTab {
     int8_t nrl[ACE_LMAX + 1];  // n bound per l (max pooled n over mu, m)
 };
@@ -1564,12 +1615,185 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
     }
 }
 
+// k_forcef: forces over the flat neighbour list (Invariant, one species,
+// nr <= 16).  Every lane is a neighbour, whatever environment it belongs to,
+// so no lane idles on short environments.  A lane reads its environment's
+// adjoint G[slot][env] straight from memory (lanes of one environment read the
+// same address; neighbouring environments are adjacent), with the slot of
+// every (l, m, n) in the kernel arguments.  Radial values stay in registers
+// and the ALP values are produced m-outer exactly as in k_force.
+
+template <int L, int NRM>
+__global__ __launch_bounds__(256) void k_forcef(RadialParams rp, ForceFlatTab ft, int64_t nnbr,
+                                                 const double* __restrict__ xyz,
+                                                 const int32_t* __restrict__ nbr_env,
+                                                 const double2* __restrict__ Gbuf, int64_t estride,
+                                                 double* __restrict__ dE) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= nnbr) return;  // no barriers below
+    const int nr = rp.nr;
+    const int64_t env = nbr_env[j];
+    const double2* Ge = Gbuf + env;
+    const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+    const Sph S = cart2spher(x, y, z);
+    double t, dtdr;
+    transform_ed(rp, S.r, t, dtdr);
+    double R[NRM], dR[NRM];
+    {
+        double p0 = rp.A[0] * fcut(rp, t), d0 = rp.A[0] * fcut_d(rp, t);
+        R[0] = p0;
+        dR[0] = d0;
+        double p1 = 0.0, d1 = 0.0;
+        if (nr > 1) {
+            const double al = rp.A[1] * t + rp.B[1];
+            p1 = al * p0;
+            d1 = al * d0 + rp.A[1] * p0;
+        }
+        R[1] = p1;
+        dR[1] = d1;
+#pragma unroll
+        for (int n = 2; n < NRM; ++n) {
+            if (n < nr) {
+                const double al = rp.A[n] * t + rp.B[n];
+                const double p2 = al * p1 + rp.C[n] * p0;
+                const double d2 = al * d1 + rp.C[n] * d0 + rp.A[n] * p1;
+                p0 = p1;
+                p1 = p2;
+                d0 = d1;
+                d1 = d2;
+            }
+            R[n] = p1;
+            dR[n] = d1;
+        }
+    }
+    const double rinv = 1.0 / (S.r + 2.220446049250313e-16);
+    const double rx = x / S.r, ry = y / S.r, rz = z / S.r;
+    const double st2 = S.sth * S.sth;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    double epr = 0.7071067811865475, epi = 0.0;
+    double Pmm = 0.3989422804014327, dPmm = 0.0;
+    const double temp0 = 0.3989422804014327;
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+        if (m == 1) {
+            Pmm = -1.224744871391589 * temp0;
+            dPmm = -1.224744871391589 * (S.cth * temp0);
+        } else if (m >= 2) {
+            const double f = sqrt(1.0 + 0.5 / m);
+            const double nt1 = -f * S.sth * Pmm;
+            const double ntd = -f * (S.cth * Pmm * S.sth + S.sth * dPmm);
+            Pmm = nt1;
+            dPmm = ntd;
+        }
+        if (m > 0) {
+            const double nr_ = epr * S.cphi - epi * S.sphi;
+            const double ni_ = epr * S.sphi + epi * S.cphi;
+            epr = nr_;
+            epi = ni_;
+        }
+        double Pa = 0.0, dPa = 0.0, Pb = Pmm, dPb = dPmm;
+#pragma unroll
+        for (int l = m; l <= L; ++l) {
+            double Pl, dPl;
+            if (l == m) {
+                Pl = Pmm;
+                dPl = dPmm;
+            } else if (l == m + 1) {
+                if (m == 0) {
+                    Pl = S.cth * 1.7320508075688772 * temp0;
+                    dPl = -S.sth * 1.7320508075688772 * temp0;
+                } else {
+                    const double c = sqrt(2.0 * m + 3.0);
+                    Pl = c * S.cth * Pmm;
+                    dPl = c * (-st2 * Pmm + S.cth * dPmm);
+                }
+            } else {
+                const int i = idxp(l, m);
+                Pl = c_alpA[i] * (S.cth * Pb + c_alpB[i] * Pa);
+                dPl = m == 0 ? c_alpA[i] * (-S.sth * Pb + S.cth * dPb + c_alpB[i] * dPa)
+                             : c_alpA[i] * (-st2 * Pb + S.cth * dPb + c_alpB[i] * dPa);
+            }
+            if (l > m) {
+                Pa = Pb;
+                dPa = dPb;
+                Pb = Pl;
+                dPb = dPl;
+            }
+            double Yr, Yi, fpr, fpi, ftr, fti;
+            if (m == 0) {
+                Yr = Pl * 0.7071067811865475;
+                Yi = 0.0;
+                fpr = fpi = 0.0;
+                ftr = dPl * 0.7071067811865475;
+                fti = 0.0;
+            } else {
+                Yr = epr * Pl * S.sth;
+                Yi = epi * Pl * S.sth;
+                fpr = -m * epi * Pl;
+                fpi = m * epr * Pl;
+                ftr = epr * dPl;
+                fti = epi * dPl;
+            }
+            const double dYr[3] = {(-S.sphi * fpr + S.cphi * S.cth * ftr) * rinv,
+                                   (S.cphi * fpr + S.sphi * S.cth * ftr) * rinv,
+                                   (-S.sth * ftr) * rinv};
+            const double dYi[3] = {(-S.sphi * fpi + S.cphi * S.cth * fti) * rinv,
+                                   (S.cphi * fpi + S.sphi * S.cth * fti) * rinv,
+                                   (-S.sth * fti) * rinv};
+            const int16_t* sl = ft.slot[idxp(l, m)];
+            const int nb = ft.nrl[l];
+#pragma unroll
+            for (int n = 0; n < NRM; ++n) {
+                if (n < nb) {
+                    const double2 G = Ge[(int64_t)sl[n] * estride];
+                    const double Rn = R[n], dRn = dR[n] * dtdr;
+                    const double gy = G.x * Yr - G.y * Yi;
+                    g0 += dRn * gy * rx + Rn * (G.x * dYr[0] - G.y * dYi[0]);
+                    g1 += dRn * gy * ry + Rn * (G.x * dYr[1] - G.y * dYi[1]);
+                    g2 += dRn * gy * rz + Rn * (G.x * dYr[2] - G.y * dYi[2]);
+                }
+            }
+        }
+    }
+    dE[j] = g0;
+    dE[nnbr + j] = g1;
+    dE[2 * nnbr + j] = g2;
+}
+
+// G[primary slot] += G[extra slot] for pooled functions whose gradient
+// polynomial was split over several tasks (k_forcef reads one slot per k).
+__global__ void k_gfold(int64_t nenv, int nfold, const int32_t* __restrict__ fold, double2* __restrict__ Gbuf,
+                        int64_t estride) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nenv) return;
+    for (int q = 0; q < nfold; ++q) {
+        double2* dst = Gbuf + (int64_t)fold[2 * q] * estride + e;
+        const double2 v = Gbuf[(int64_t)fold[2 * q + 1] * estride + e];
+        dst->x += v.x;
+        dst->y += v.y;
+    }
+}
+
 template <int L, int PROP>
 static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
                          const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride,
                          double* dE) {
     constexpr int NC = PROP == 0 ? 1 : 3;
     constexpr int NP = (L + 1) * (L + 2) / 2;
+    if (PROP == 0 && ctx->flat_force) {
+        if (ctx->nfold > 0)
+            hipLaunchKernelGGL(k_gfold, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, ctx->nfold,
+                               ctx->gfold.p, ctx->Gbuf.p, estride);
+        // radial functions held in registers: 12 when they suffice (fewer
+        // registers, more waves per SIMD), else 16
+        if (ctx->nr <= 12)
+            hipLaunchKernelGGL((k_forcef<L, 12>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), 0, st,
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->Gbuf.p, estride, dE);
+        else
+            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), 0, st,
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->Gbuf.p, estride, dE);
+        return;
+    }
     const int nsp = std::max(1, ctx->nspecies);
     const size_t dense = (size_t)nsp * NP * ctx->nr * NC * sizeof(double2);
     if (ctx->nr <= FORCE_NRMAX && dense <= 64 * 1024) {

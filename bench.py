@@ -26,9 +26,9 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_dagr launch on this workload (100k environments, 16-wave
 # tiles), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script,
 # scaled by the factors a 256 MiB device copy gives the same counters (2.0 for
-# FETCH_SIZE, 1.0 for WRITE_SIZE): 570.9 MB read + 224.8 MB written.  Only
+# FETCH_SIZE, 1.0 for WRITE_SIZE): 259.4 MB read + 223.2 MB written.  Only
 # valid for the default --nenv; re-measured whenever the kernel changes.
-DAG_TRAFFIC_BYTES = {100_000: 795.7e6}
+DAG_TRAFFIC_BYTES = {100_000: 482.6e6}
 TRAFFIC_SOURCE = "profiles/plain_ef_config2_kernels.md"
 
 
