@@ -90,6 +90,10 @@ class HIPEvaluator:
         _lib.check(self.lib.ace_set_tile(self.ctx, int(waves)))
         self.waves = int(waves)
 
+    def set_parts(self, parts: int = 0):
+        """Workgroups per product-DAG tile: 0 chooses per call (default), else 1, 2 or 4."""
+        _lib.check(self.lib.ace_set_parts(self.ctx, int(parts)))
+
     def set_validation(self, on: bool):
         _lib.check(self.lib.ace_set_validation(self.ctx, int(bool(on))))
 

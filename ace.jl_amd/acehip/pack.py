@@ -15,7 +15,7 @@ from . import programs as P
 WAVE_CONFIGS = (4, 8, 16)     # waves per workgroup of the product-DAG kernel
 ENERGY_PARTS = 16             # energy polynomial split into this many tasks
 REC_LEAVES = 4                # leaves per 64-byte program record
-MAX_TASK_COST = 6000.0        # gradient polynomials above this are split
+MAX_TASK_COST = 1200.0        # gradient polynomials above this are split (tasks fit a quarter-tile wave)
 
 
 class Pack:

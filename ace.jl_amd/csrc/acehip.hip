@@ -438,10 +438,11 @@ struct DagRecArgs {
     const uint32_t* recs;      // [nrec + 1][16]
     const int32_t* tasks;      // [ntasks][5]
     const double* task_const;  // [ntasks][2]
-    const int32_t* wtask_ptr;  // [W+1]
+    const int32_t* wtask_ptr;  // [parts * W + 1]
     const int32_t* wtask_list;
     int npool;
     int W;
+    int parts;                 // workgroups per tile, each running 1/parts of the tasks
 };
 
 // Streaming accesses: the A+ tile is read once and G written once per
@@ -523,12 +524,15 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
     extern __shared__ double2 As[];  // [npool][64]
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int64_t env = (int64_t)blockIdx.x * 64 + lane;
+    // workgroup = (tile, part): the parts of a tile run disjoint task lists
+    const int64_t tile = blockIdx.x / a.parts;
+    const int part = (int)(blockIdx.x - tile * a.parts);
+    int64_t env = tile * 64 + lane;
     bool valid = env < nenv;
     if (env_list) {
         // environments listed by the caller (fallback of the learned surrogate)
         const int64_t cnt = *env_count;
-        if ((int64_t)blockIdx.x * 64 >= cnt) return;  // whole workgroup idle: before any barrier
+        if (tile * 64 >= cnt) return;  // whole workgroup idle: before any barrier
         valid = env < cnt;
         env = valid ? env_list[env] : 0;
     }
@@ -538,7 +542,7 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
 
     const char* lds = reinterpret_cast<const char*>(As) + 16 * lane;
     double ep = 0.0;
-    const int tb = a.wtask_ptr[wave], te = a.wtask_ptr[wave + 1];
+    const int tb = a.wtask_ptr[part * a.W + wave], te = a.wtask_ptr[part * a.W + wave + 1];
     for (int ti = tb; ti < te; ++ti) {
         const int t = a.wtask_list[ti];
         const int slot = a.tasks[5 * t + 2], rb = a.tasks[5 * t + 3], re = a.tasks[5 * t + 4];
@@ -580,10 +584,21 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
     red[wave * 64 + lane] = ep;
     __syncthreads();
     if (wave == 0 && valid && E_out) {
-        double s = e0r;
+        // one part: the energy; several: this part's share, summed by k_esum
+        double s = a.parts == 1 ? e0r : 0.0;
         for (int w = 0; w < a.W; ++w) s += red[w * 64 + lane];
-        E_out[env] = s;
+        E_out[(int64_t)part * estride + env] = s;
     }
+}
+
+// E = e0 + sum of the parts' energy shares (fixed order: deterministic).
+__global__ void k_esum(int64_t nenv, int parts, const double* __restrict__ Ep, int64_t estride, double e0,
+                       double* __restrict__ E_out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nenv) return;
+    double s = e0;
+    for (int p = 0; p < parts; ++p) s += Ep[(int64_t)p * estride + e];
+    E_out[e] = s;
 }
 
 
@@ -1095,10 +1110,14 @@ struct DevBuf {
     }
 };
 
+constexpr int MAX_PARTS = 4;  // workgroups per product-DAG tile: 1, 2 or 4
+
 struct WaveCfg {
     int W;
     std::vector<int32_t> ptr_all, list_all, ptr_e, list_e;
     DevBuf<int32_t> d_ptr_all, d_list_all, d_ptr_e, d_list_e;
+    // task lists over parts * W virtual waves (Invariant), parts = 2 and 4
+    DevBuf<int32_t> pp_all[MAX_PARTS + 1], pl_all[MAX_PARTS + 1], pp_e[MAX_PARTS + 1], pl_e[MAX_PARTS + 1];
 };
 
 constexpr int FORCE_NRMAX = 16;  // radial functions held in registers by the force kernels
@@ -1139,7 +1158,11 @@ struct ace_ctx {
     double e0r = 0.0;
     std::vector<WaveCfg*> wcfg;
     int cur_w = 0;
+    int ncu = 256;  // compute units of the device (tail sizing of the product-DAG grid)
+    int parts = 0;  // workgroups per product-DAG tile: 0 = chosen per call, else 1, 2 or 4
+    int last_parts = 1;
     // scratch
+    DevBuf<double> Epart;
     DevBuf<double2> Aplus, Gbuf, Afull, AA, dphi, dAA;
     DevBuf<int32_t> nbr_env;
     // timing
@@ -1216,8 +1239,11 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
     if (!werr.empty()) return set_err("ace_ctx_create: " + werr);
     HIP_TRY(hipSetDevice(device));
     if (ensure_alp(device)) return -1;
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
     ace_ctx* ctx = new ace_ctx();
     ctx->device = device;
+    ctx->ncu = std::max(1, ncu);
     ctx->prop = pk->prop;
     ctx->L = pk->L;
     ctx->nr = pk->nr;
@@ -1316,6 +1342,18 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             wc->d_ptr_e.upload(wc->ptr_e.data(), wc->ptr_e.size()) ||
             wc->d_list_e.upload(wc->list_e.data(), wc->list_e.size()))
             return fail("upload");
+        if (pk->prop == ACE_PROP_INVARIANT) {
+            const std::vector<int> t_all = acehost::mode_tasks(ctx->host, true), t_e = acehost::mode_tasks(ctx->host, false);
+            for (int P = 2; P <= MAX_PARTS; P *= 2) {
+                std::vector<int32_t> ptr, list;
+                acehost::lpt_lists(ctx->host, t_all, P * t.W, ptr, list);
+                if (wc->pp_all[P].upload(ptr.data(), ptr.size()) || wc->pl_all[P].upload(list.data(), list.size()))
+                    return fail("upload");
+                acehost::lpt_lists(ctx->host, t_e, P * t.W, ptr, list);
+                if (wc->pp_e[P].upload(ptr.data(), ptr.size()) || wc->pl_e[P].upload(list.data(), list.size()))
+                    return fail("upload");
+            }
+        }
     }
     ctx->nfeat = pk->nfeat;
     if (pk->nfeat > 0) {
@@ -1339,6 +1377,13 @@ extern "C" int ace_set_params(ace_ctx* ctx, const double* c, int64_t nB) {
     if (nB != ctx->nB) return set_err("ace_set_params: len(c) != len(basis)");
     HIP_TRY(hipSetDevice(ctx->device));
     return upload_coeffs(ctx, c);
+}
+
+extern "C" int ace_set_parts(ace_ctx* ctx, int parts) {
+    if (!ctx) return set_err("ace_set_parts: null context");
+    if (parts != 0 && parts != 1 && parts != 2 && parts != 4) return set_err("ace_set_parts: parts must be 0, 1, 2 or 4");
+    ctx->parts = parts;
+    return 0;
 }
 
 extern "C" int ace_set_tile(ace_ctx* ctx, int W) {
@@ -1869,18 +1914,45 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     if (ctx->prop == ACE_PROP_INVARIANT && lds > 160 * 1024)
         return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     if (ctx->prop == ACE_PROP_INVARIANT) {
+        // Parts per tile: the grid runs in rounds of ncu workgroups, and the last
+        // round may be nearly empty; splitting each tile's tasks over P
+        // workgroups shortens the rounds.  Pick P in {1, 2, 4} minimising
+        // rounds(P) / P, with 2 % per extra part for re-staging the A+ tile.
+        // A split is only considered while every virtual wave keeps at least 4
+        // tasks (fewer leave the part workgroups unbalanced or idle).
+        int P = ctx->parts;
+        const int ntask_mode = forces ? wc->ptr_all[wc->W] : wc->ptr_e[wc->W];
+        int pmax = 1;
+        while (pmax < MAX_PARTS && ntask_mode >= 4 * (2 * pmax) * wc->W) pmax *= 2;
+        if (P == 0) {
+            double best = 1e300;
+            for (int q = 1; q <= pmax; q *= 2) {
+                const double rounds = (double)((ntile * q + ctx->ncu - 1) / ctx->ncu);
+                const double cost = rounds / q * (1.0 + 0.02 * (q - 1));
+                if (cost < best - 1e-12) {
+                    best = cost;
+                    P = q;
+                }
+            }
+        }
+        ctx->last_parts = P;
         DagRecArgs ra;
         ra.recs = ctx->recs.p;
         ra.tasks = ctx->d_tasks.p;
         ra.task_const = ctx->task_const.p;
-        ra.wtask_ptr = a.wtask_ptr;
-        ra.wtask_list = a.wtask_list;
+        ra.wtask_ptr = P == 1 ? a.wtask_ptr : (forces ? wc->pp_all[P].p : wc->pp_e[P].p);
+        ra.wtask_list = P == 1 ? a.wtask_list : (forces ? wc->pl_all[P].p : wc->pl_e[P].p);
         ra.npool = ctx->npool;
         ra.W = wc->W;
+        ra.parts = P;
+        if (P > 1 && ctx->Epart.ensure((size_t)P * estride)) return -1;
         set_func_lds((const void*)k_dagr, lds);
-        hipLaunchKernelGGL(k_dagr, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, ra, nenv,
-                           ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r, (const int32_t*)nullptr,
-                           (const int32_t*)nullptr);
+        hipLaunchKernelGGL(k_dagr, dim3((unsigned)(ntile * P)), dim3(64 * wc->W), lds, st, ra, nenv,
+                           ctx->Aplus.p, estride, P == 1 ? E_out : ctx->Epart.p, ctx->Gbuf.p, ctx->e0r,
+                           (const int32_t*)nullptr, (const int32_t*)nullptr);
+        if (P > 1)
+            hipLaunchKernelGGL(k_esum, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, P, ctx->Epart.p,
+                               estride, ctx->e0r, E_out);
     } else if ((size_t)ctx->npool * 64 * sizeof(double2) <= 160 * 1024) {
         set_func_lds((const void*)k_dag<1, 64>, lds);
         hipLaunchKernelGGL((k_dag<1, 64>), dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
@@ -2132,6 +2204,7 @@ extern "C" int ace_surr_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t*
     ra.wtask_list = forces ? wc->d_list_all.p : wc->d_list_e.p;
     ra.npool = ctx->npool;
     ra.W = wc->W;
+    ra.parts = 1;
     size_t lds = std::max((size_t)ctx->npool * 64 * sizeof(double2), (size_t)wc->W * 64 * sizeof(double));
     if (lds > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     set_func_lds((const void*)k_dagr, lds);

@@ -281,6 +281,37 @@ inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint
     return "";
 }
 
+// Task lists of the two kernel modes: every task (energy + forces), energy only.
+inline std::vector<int> mode_tasks(const PackCopy& p, bool forces) {
+    std::vector<int> out;
+    for (int t = 0; t < p.ntasks; ++t)
+        if (forces || p.tasks[5 * (size_t)t + 2] < 0) out.push_back(t);
+    return out;
+}
+
+// Longest-first assignment of tasks to V virtual waves by record count (each
+// to the least loaded wave), every list ascending: ptr[V + 1], list.
+inline void lpt_lists(const PackCopy& p, const std::vector<int>& tasks, int V, std::vector<int32_t>& ptr,
+                      std::vector<int32_t>& list) {
+    auto cost = [&](int t) { return std::max(1, p.tasks[5 * (size_t)t + 4] - p.tasks[5 * (size_t)t + 3]); };
+    std::vector<int> order(tasks);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    std::vector<long> load(V, 0);
+    std::vector<std::vector<int>> lists(V);
+    for (int t : order) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[w].push_back(t);
+        load[w] += cost(t);
+    }
+    ptr.assign(1, 0);
+    list.clear();
+    for (auto& l : lists) {
+        std::sort(l.begin(), l.end());
+        list.insert(list.end(), l.begin(), l.end());
+        ptr.push_back((int32_t)list.size());
+    }
+}
+
 // (mu, l, m, n) -> pooled index, -1 when absent (force kernel lookup).
 inline std::vector<int16_t> pool_lookup(const ace_pack* pk) {
     const int ns = std::max(1, (int)pk->nspecies);

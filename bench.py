@@ -24,11 +24,12 @@ METRIC = "env/s energy+forces: plain FP64 vs surrogate at ≤1 meV/atom MAE"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_dagr launch on this workload (100k environments, 16-wave
-# tiles), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script,
-# scaled by the factors a 256 MiB device copy gives the same counters (2.0 for
-# FETCH_SIZE, 1.0 for WRITE_SIZE): 259.4 MB read + 223.2 MB written.  Only
-# valid for the default --nenv; re-measured whenever the kernel changes.
-DAG_TRAFFIC_BYTES = {100_000: 482.6e6}
+# tiles split over 2 workgroups), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script, scaled by the factors a 256 MiB device copy
+# gives the same counters (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE): 550.4 MB
+# read + 269.1 MB written.  Only valid for the default --nenv; re-measured
+# whenever the kernel changes.
+DAG_TRAFFIC_BYTES = {100_000: 819.6e6}
 TRAFFIC_SOURCE = "profiles/plain_ef_config2_kernels.md"
 
 
@@ -228,8 +229,9 @@ def main():
                                "order 4, degree 12, Lmax 6, Invariant, FP64",
                    "nenv": b.nenv, "nnbr": b.nnbr, "order": 4, "maxdeg": 12, "L": 6,
                    "len_B": len(basis), "len_AA": len(pk.orders), "npool": len(pk.pool),
-                   "tile_waves": ev.waves,
-                   "autotune_ms": None if tuned is None else {str(k): round(v, 3) for k, v in tuned[1].items()}},
+                   "tile_waves": ev.waves, "tile_parts": None if tuned is None else tuned[0][1],
+                   "autotune_ms": None if tuned is None else {f"{k[0]}x{k[1]}": round(v, 3)
+                                                              for k, v in tuned[1].items()}},
         "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (no MFMA); MI355X FP64 peak",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),

@@ -144,6 +144,11 @@ int ace_set_params(ace_ctx* ctx, const double* c, int64_t nB);
  * the pack's wave configurations). */
 int ace_set_tile(ace_ctx* ctx, int waves_per_tile);
 
+/* Workgroups per product-DAG tile (Invariant): each runs 1/parts of the
+ * tile's tasks, which shortens the last, partly empty round of workgroups.
+ * 0 (default) chooses 1, 2 or 4 per call from the tile and CU counts. */
+int ace_set_parts(ace_ctx* ctx, int parts);
+
 /* Site energies, ProductEvaluator evaluate (evaluator.jl:121-147), batched.
  * E_out[nenv] (Invariant) or E_out[nenv][3] (EuclideanVector).  species
  * [nnbr] int8 category index (NULL without a categorical 1p basis). */

@@ -18,8 +18,9 @@ E_TOL = 1e-12
 F_TOL = 1e-10
 
 
-def _run(basis, T, c, off, R, sp=None, waves=8, forces=True):
+def _run(basis, T, c, off, R, sp=None, waves=8, forces=True, parts=0):
     ev = HIPEvaluator(basis, c, device=0, waves=waves)
+    ev.set_parts(parts)
     b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
     if forces:
         E, dE = ev.energy_forces(b)
@@ -41,6 +42,28 @@ def test_config1_energy_forces(cuda, waves):
     assert rel_err(E, E_ref) < E_TOL
     g_ref = O.grad_config(T, c, off, R)
     assert rel_err(dE.T, g_ref) < F_TOL
+
+
+@pytest.mark.parametrize("parts", [1, 2, 4])
+def test_config2_tile_parts(cuda, parts):
+    """Each tile's tasks split over 1, 2 or 4 workgroups: energies summed from
+    the parts' shares, gradient tasks each in exactly one part."""
+    print(f"[gpu] order-4 degree-12 basis, parts={parts}", flush=True)
+    basis, T = basis_for(4, 12, 6)
+    c = synth.coefficients(len(basis), 2)
+    off, R, sp = synth.workload_envs(2, nenv=130)
+    for forces in (True, False):
+        _, E, dE = _run(basis, T, c, off, R, waves=16, forces=forces, parts=parts)
+        assert rel_err(E, O.energy(T, c, off, R)) < E_TOL
+        if forces:
+            assert rel_err(dE.T, O.grad_config(T, c, off, R)) < F_TOL
+
+
+def test_set_parts_rejects_bad_values(cuda):
+    basis, T = basis_for(3, 6, 4)
+    ev = HIPEvaluator(basis, synth.coefficients(len(basis), 1), device=0)
+    with pytest.raises(Exception):
+        ev.set_parts(3)
 
 
 def test_config1_energy_only(cuda):
