@@ -1160,6 +1160,9 @@ struct ace_ctx {
     int cur_w = 0;
     int ncu = 256;  // compute units of the device (tail sizing of the product-DAG grid)
     int parts = 0;  // workgroups per product-DAG tile: 0 = chosen per call, else 1, 2 or 4
+    // last eagerly evaluated batch: its neighbour count serves calls under stream capture
+    const int32_t* batch_off = nullptr;
+    int64_t batch_nenv = -1, batch_nnbr = 0;
     int last_parts = 1;
     // scratch
     DevBuf<double> Epart;
@@ -1425,19 +1428,34 @@ static RadialParams radial_params(ace_ctx* ctx) {
 }
 
 static int check_common(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
-                        const int8_t* species, int64_t* nnbr_out) {
+                        const int8_t* species, int64_t* nnbr_out, void* stream) {
     if (!ctx) return set_err("null context");
     if (nenv < 0) return set_err("nenv < 0");
     if (nenv > 0 && (!off || !xyz)) return set_err("null env_offsets / xyz");
     if (ctx->nspecies > 0 && nenv > 0 && !species)
         return set_err("species array required for a categorical 1p basis");
     HIP_TRY(hipSetDevice(ctx->device));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing((hipStream_t)stream, &cap));
+    if (cap != hipStreamCaptureStatusNone) {
+        // stream capture (hipGraph): no host reads; the neighbour count comes
+        // from the eager call on the same batch that must precede the capture
+        // (it also sizes every scratch buffer)
+        if (nenv > 0 && (off != ctx->batch_off || nenv != ctx->batch_nenv))
+            return set_err("stream capture needs an eager call on the same batch (env_offsets, nenv) first");
+        if (ctx->validate) return set_err("stream capture needs validation off (ace_set_validation(ctx, 0))");
+        *nnbr_out = nenv > 0 ? ctx->batch_nnbr : 0;
+        return 0;
+    }
     int32_t last = 0;
     if (nenv > 0) {
         int32_t first = 0;
         HIP_TRY(hipMemcpy(&first, off, sizeof(int32_t), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&last, off + nenv, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (first != 0 || last < 0) return set_err("env_offsets must start at 0 and be non-negative");
+        ctx->batch_off = off;
+        ctx->batch_nenv = nenv;
+        ctx->batch_nnbr = last;
     }
     *nnbr_out = last;
     return 0;
@@ -1883,7 +1901,7 @@ static void launch_force_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const i
 static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
                     const int8_t* species, double* E_out, double* dE, void* stream) {
     int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
     if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
     if (nenv == 0) return 0;
     if (!E_out) return set_err("null E_out");
@@ -1895,10 +1913,14 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     const bool forces = dE != nullptr;
     if (forces && ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * NC * estride)) return -1;
     WaveCfg* wc = ctx->wcfg[ctx->cur_w];
-    HIP_TRY(hipEventRecord(ctx->ev[0], st));
+    // under stream capture (hipGraph) the timing events are left out of the graph
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(st, &cap));
+    const bool timed = cap == hipStreamCaptureStatusNone;
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[0], st));
     DISPATCH_L(ctx->L, launch_pool, ctx, st, nenv, off, xyz, nnbr, species, estride);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[1], st));
     DagArgs a;
     a.runs = ctx->runs.p;
     a.leaf_fid = ctx->leaf_fid.p;
@@ -1969,13 +1991,13 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
                            ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0r, ctx->e0v.p);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[2], st));
     if (forces && nnbr > 0) {
         DISPATCH_L(ctx->L, launch_force_any, ctx, st, nenv, off, xyz, nnbr, species, estride, dE);
         HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipEventRecord(ctx->ev[3], st));
-    ctx->timing_valid = true;
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    ctx->timing_valid = timed;
     return 0;
 }
 
@@ -2025,7 +2047,7 @@ static int basis_A_internal(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
 extern "C" int ace_basis_A(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
                            const int8_t* species, double* A_out, void* stream) {
     int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
     if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
     if (nenv == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
@@ -2052,7 +2074,7 @@ __global__ void k_realpart(int64_t n, const double2* __restrict__ in, double* __
 extern "C" int ace_basis_AA(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
                             const int8_t* species, double* AA_out, void* stream) {
     int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
     if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
     if (nenv == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
@@ -2071,7 +2093,7 @@ extern "C" int ace_basis_AA(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
 extern "C" int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
                            const int8_t* species, double* B_out, void* stream) {
     int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
     if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
     if (nenv == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
@@ -2099,7 +2121,7 @@ static void launch_dphi(ace_ctx* ctx, hipStream_t st, int64_t nnbr, const double
 extern "C" int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
                             const int8_t* species, double* dB_out, void* stream) {
     int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
     if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
     if (nenv == 0 || nnbr == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
@@ -2147,7 +2169,7 @@ extern "C" int ace_surr_features(ace_ctx* ctx, int64_t nenv, const int32_t* off,
                                  const int8_t* species, const double* xmean, const double* xinvstd, double* X,
                                  void* Xn, double* E2, void* stream) {
     int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
     if (ctx->prop != ACE_PROP_INVARIANT || ctx->nfeat < 1) return set_err("surrogate features need an Invariant basis");
     if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
     if (nenv == 0) return 0;
@@ -2170,7 +2192,7 @@ extern "C" int ace_surr_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t*
                                       const double* E2, double* E_out, double* dE, int32_t* fb_count,
                                       int32_t* fb_list, void* stream) {
     int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr)) return -1;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
     if (ctx->prop != ACE_PROP_INVARIANT || ctx->nfeat < 1) return set_err("surrogate needs an Invariant basis");
     if (nenv == 0) return 0;
     if (ctx->surr_nenv != nenv) return set_err("ace_surr_energy_forces must follow ace_surr_features on the same batch");

@@ -146,6 +146,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--waves", type=int, default=0, help="tile waves (0 = autotune)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay timing")
     ap.add_argument("--train-envs", type=int, default=1_000_000)
     ap.add_argument("--eval-envs", type=int, default=1_000_000)
     ap.add_argument("--epochs", type=int, default=12)
@@ -198,6 +199,27 @@ def main():
     dt = time.perf_counter() - t0
     ms_step = dt / args.steps * 1e3
     value = b.nenv * args.steps / dt
+    # the same step captured once as a hipGraph (torch's graph API over HIP
+    # stream capture) and replayed: launch overhead removed, same kernels
+    graph_ms = None
+    if not args.no_graph:
+        gs = torch.cuda.Stream()
+        gs.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(gs):
+            ev.energy_forces(b, E, dE)
+        torch.cuda.current_stream().wait_stream(gs)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            ev.energy_forces(b, E, dE)
+        graph.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph.replay()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        del graph
     # copy kernel of known size for the counter calibration runs
     if args.copy_calib_mb > 0:
         nb = args.copy_calib_mb * (1 << 20) // 8
@@ -242,6 +264,7 @@ def main():
                      "executed_tflops": round(achieved_exe, 3),
                      "executed_frac": round(achieved_exe / FP64_PEAK_TFLOPS, 4),
                      "pool_ms": round(float(np.mean(pool_ms)), 4),
+                     "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
                      "force_ms": round(float(np.mean(force_ms)), 4)},
     }
     if not args.no_surrogate:
