@@ -50,7 +50,10 @@ EXPORTS = ("ace_ctx_create", "ace_set_params", "ace_set_tile", "ace_set_parts", 
            "ace_energy_forces", "ace_basis_A", "ace_basis_AA", "ace_basis_B",
            "ace_basis_dB", "ace_surr_features", "ace_surr_energy_forces",
            "ace_last_timing", "ace_set_validation", "ace_last_error",
-           "ace_ctx_destroy", "ace_abi_version")
+           "ace_ctx_destroy", "ace_abi_version", "ace_nbrlist_create", "ace_nbrlist_destroy",
+           "ace_nbrlist_build", "ace_nbrlist_update", "ace_nbrlist_forces")
+
+ERR_CAPACITY = -2  # ACE_ERR_CAPACITY
 
 _lib = None
 
@@ -88,6 +91,13 @@ def load(path: str | None = None):
     lib.ace_ctx_destroy.argtypes = [vp]
     lib.ace_ctx_destroy.restype = None
     lib.ace_abi_version.restype = C.c_int
+    lib.ace_nbrlist_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.ace_nbrlist_destroy.argtypes = [vp]
+    lib.ace_nbrlist_destroy.restype = None
+    lib.ace_nbrlist_build.argtypes = [vp, C.c_int64, vp, _f64p, _i32p, C.c_double, vp, C.c_int64, vp, vp, vp, vp,
+                                      vp, vp, C.POINTER(C.c_int64), vp]
+    lib.ace_nbrlist_update.argtypes = [vp, C.c_int64, vp, _f64p, C.c_int64, vp, vp, vp, vp, vp]
+    lib.ace_nbrlist_forces.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp]
     if lib.ace_abi_version() != ABI_VERSION:
         raise AceError("libacehip.so ABI version mismatch")
     if path is None:

@@ -95,6 +95,16 @@ def fcc_envs(nenv, seed, a=3.6, ncell=7, sigma=0.1, rcut=5.0):
     return np.asarray(offs, dtype=np.int32), np.concatenate(Rs, axis=0), None
 
 
+def fcc_crystal(ncells=(30, 30, 28), a=3.6, sigma=0.1, seed=0):
+    """Positions (N, 3) and cubic cell (3, 3) of a periodic FCC crystal of
+    ``ncells`` conventional cells with Gaussian displacements ``sigma``."""
+    basis = 0.5 * a * np.array([[0, 0, 0], [0, 1, 1], [1, 0, 1], [1, 1, 0]], dtype=np.float64)
+    g = np.stack(np.meshgrid(*[np.arange(k) for k in ncells], indexing="ij"), -1).reshape(-1, 3)
+    x = (g[:, None, :] * a + basis[None]).reshape(-1, 3)
+    x = x + sigma * np.random.default_rng(seed).standard_normal(x.shape)
+    return x, np.diag(np.asarray(ncells, dtype=np.float64) * a)
+
+
 def workload_envs(wid: int, nenv: int | None = None, seed: int | None = None):
     w = WORKLOADS[wid]
     nenv = w["nenv"] if nenv is None else nenv

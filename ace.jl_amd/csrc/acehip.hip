@@ -32,29 +32,13 @@
 #include <vector>
 
 #include "../../include/acehip.h"
+#include "host_util.hpp"
 #include "pack_host.hpp"
 
 #define ACE_LMAX 10
 #define ACE_NPMAX ((ACE_LMAX + 1) * (ACE_LMAX + 2) / 2)
 #define ACE_NRMAX 32
 #define ACE_CONJ 0x8000
-
-// ---------------------------------------------------------------------------
-// error handling
-
-static thread_local std::string g_err;
-
-static int set_err(const std::string& s) {
-    g_err = s;
-    return -1;
-}
-
-#define HIP_TRY(expr)                                                              \
-    do {                                                                           \
-        hipError_t e_ = (expr);                                                    \
-        if (e_ != hipSuccess)                                                      \
-            return set_err(std::string(#expr) + ": " + hipGetErrorString(e_));    \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // universal ALP recursion coefficients (sphericalharmonics.jl:146-159)
@@ -1084,31 +1068,6 @@ __global__ __launch_bounds__(256) void k_feat_grad(FeatArgs a, int64_t nenv, con
 
 // ---------------------------------------------------------------------------
 // host side
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    int ensure(size_t count) {
-        if (count <= n) return 0;
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-            n = 0;
-        }
-        HIP_TRY(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
-        n = count;
-        return 0;
-    }
-    int upload(const T* h, size_t count) {
-        if (ensure(count)) return -1;
-        if (count) HIP_TRY(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
-        return 0;
-    }
-};
 
 constexpr int MAX_PARTS = 4;  // workgroups per product-DAG tile: 1, 2 or 4
 

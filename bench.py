@@ -137,6 +137,57 @@ def surrogate_leg(ev, b, E_plain, steps, warmup, ntrain, neval, epochs, log):
                       "seconds": round(info["seconds"], 1), "final_loss": info["loss"][-1]}}
 
 
+def md_leg(ev, steps, warmup):
+    """Whole-configuration step on a 100 800-atom FCC crystal (rcut 5 A):
+    neighbour-list build, then per step update -> energy_forces -> force
+    assembly, each timed with HIP events on the launch stream."""
+    import torch
+    from acehip import synth
+    from acehip.atoms import NeighbourList
+
+    x, cell = synth.fcc_crystal((30, 30, 28))
+    dev = torch.device("cuda:0")
+    pos = torch.as_tensor(x.T.copy(), device=dev)
+    nl = NeighbourList(0)
+    build = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nl.build(pos, cell, (1, 1, 1), 5.0)
+        build.append((time.perf_counter() - t0) * 1e3)
+    b = nl.batch()
+    n, m = nl.natoms, nl.nnbr
+    E = torch.empty(n, dtype=torch.float64, device=dev)
+    dE = torch.empty((3, m), dtype=torch.float64, device=dev)
+    F = torch.empty((3, n), dtype=torch.float64, device=dev)
+    ev.set_validation(True)
+    ev.energy_forces(b, E, dE)
+    ev.set_validation(False)
+    stream = torch.cuda.current_stream()
+    ev_ = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    t_up, t_ef, t_f, t_step = [], [], [], []
+    for k in range(warmup + steps):
+        ev_[0].record(stream)
+        nl.update(pos)
+        ev_[1].record(stream)
+        ev.energy_forces(b, E, dE)
+        ev_[2].record(stream)
+        nl.forces(dE, F)
+        ev_[3].record(stream)
+        ev_[3].synchronize()
+        if k >= warmup:
+            t_up.append(ev_[0].elapsed_time(ev_[1]))
+            t_ef.append(ev_[1].elapsed_time(ev_[2]))
+            t_f.append(ev_[2].elapsed_time(ev_[3]))
+            t_step.append(ev_[0].elapsed_time(ev_[3]))
+    step = float(sum(t_step) / len(t_step))
+    avg = lambda v: round(float(sum(v) / len(v)), 4)
+    return {"workload": "FCC crystal 30x30x28 cells, a 3.6 A, 0.1 A jitter, periodic, rcut 5 A, config-2 basis",
+            "atoms": n, "nnbr": m, "build_ms": round(min(build), 3), "update_ms": avg(t_up),
+            "energy_forces_ms": avg(t_ef), "assemble_ms": avg(t_f), "step_ms": round(step, 4),
+            "atom_steps_per_s": round(n / step * 1e3, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +198,7 @@ def main():
     ap.add_argument("--waves", type=int, default=0, help="tile waves (0 = autotune)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay timing")
+    ap.add_argument("--no-md", action="store_true", help="skip the whole-configuration (neighbour list) leg")
     ap.add_argument("--train-envs", type=int, default=1_000_000)
     ap.add_argument("--eval-envs", type=int, default=1_000_000)
     ap.add_argument("--epochs", type=int, default=12)
@@ -267,6 +319,8 @@ def main():
                      "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
                      "force_ms": round(float(np.mean(force_ms)), 4)},
     }
+    if not args.no_md:
+        out["md_step"] = md_leg(ev, args.steps, args.warmup)
     if not args.no_surrogate:
         E_plain = E.clone()
         sur = surrogate_leg(ev, b, E_plain, args.steps, args.warmup, args.train_envs, args.eval_envs,

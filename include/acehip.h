@@ -228,6 +228,64 @@ const char* ace_last_error(void);
 void ace_ctx_destroy(ace_ctx* ctx);
 int ace_abi_version(void);
 
+/*
+ * Environment extraction and force assembly: the atoms-level layer around the
+ * evaluator.  ACE.jl evaluates one environment at a time; its callers build
+ * the environments of a configuration and sum site energies and forces
+ * (energy(V, at) / forces(V, at) in src/testing/testlsq.jl:27-35,42-69 and
+ * src/testing/testmodel.jl:64-81; the neighbour list itself comes from the
+ * JuLIP package, which ACE.jl v0.12.46 does not depend on).  Entry points
+ * below replace that layer for whole configurations on the device.
+ *
+ * Positions are SoA pos[3][natoms] (device).  cell[9] (host) holds the
+ * lattice vectors as rows, pbc[3] (host) 1 = periodic along that vector (the
+ * cell must be non-singular also along open axes: there it only sets the
+ * direction of the binning).  Entry m of atom i is the neighbour j at
+ * r_m = (x_j - x_i) + S'_m . cell, 0 < |r_m| < rcut (self pairs excluded,
+ * self images included), with integer image shift S'_m relative to the
+ * given positions.  The list is exactly symmetric: (i -> j, S') present iff
+ * (j -> i, -S') present, with the exactly negated vector.  The order within an
+ * environment is deterministic.
+ */
+#define ACE_ERR_CAPACITY (-2)
+
+typedef struct ace_nbrlist ace_nbrlist;
+
+int ace_nbrlist_create(int device, ace_nbrlist** out);
+void ace_nbrlist_destroy(ace_nbrlist* nl);
+
+/* Builds the neighbour lists: env_offsets[natoms+1], and with
+ * nnbr = env_offsets[natoms] entries, nbr_j[nnbr], shift[3][nnbr] (SoA, int32),
+ * xyz[3][nnbr] (the evaluator's neighbour positions, stride nnbr), rev[nnbr]
+ * (entry index of the reverse pair; nullable) and nbr_species[nnbr] =
+ * atom_species[nbr_j] (nullable; int8 category index).  The arrays must hold
+ * ``capacity`` entries (xyz and shift 3 * capacity).  Returns
+ * ACE_ERR_CAPACITY with *nnbr_out = the entries needed when capacity is too
+ * small (nothing else written).  Synchronizes ``stream``: the binning runs on
+ * the host and the list length is returned on the host. */
+int ace_nbrlist_build(ace_nbrlist* nl, int64_t natoms, const double* pos, const double* cell,
+                      const int32_t* pbc, double rcut, const int8_t* atom_species, int64_t capacity,
+                      int32_t* env_offsets, int32_t* nbr_j, int32_t* shift, double* xyz, int32_t* rev,
+                      int8_t* nbr_species, int64_t* nnbr_out, void* stream);
+
+/* Recomputes xyz[3][nnbr] = (x_j - x_i) + S' . cell for a list from
+ * ace_nbrlist_build and new positions (and cell, passed by value: a captured
+ * call keeps the cell it was captured with).  Stream-ordered, allocates
+ * nothing, capturable in a HIP graph. */
+int ace_nbrlist_update(ace_nbrlist* nl, int64_t natoms, const double* pos, const double* cell,
+                       int64_t nnbr, const int32_t* env_offsets, const int32_t* nbr_j,
+                       const int32_t* shift, double* xyz, void* stream);
+
+/* Atomic forces F[3][natoms] = -dE/dx (E = sum of site energies) from the
+ * per-neighbour gradients dE_dr[3][nnbr] of ace_energy_forces (same list),
+ * and optionally the virial[9] = -sum_m dE_dr_m r_m^T (row-major,
+ * device; needs xyz).  Deterministic (no atomics).  Stream-ordered; with a
+ * virial the first call per size allocates a small partials buffer, later
+ * calls of equal or smaller size allocate nothing. */
+int ace_nbrlist_forces(ace_nbrlist* nl, int64_t natoms, const int32_t* env_offsets, const int32_t* rev,
+                       int64_t nnbr, const double* xyz, const double* dE_dr, double* F, double* virial,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,3 +56,66 @@ for args in [(3, 6, 4, "Invariant", 0), (3, 6, 4, "EuclideanVector", 0), (3, 6, 
     s3.runs = keep3[-1].ctypes.data_as(C.POINTER(C.c_int32))
     assert lib.acehost_validate(C.byref(s3), msg, 256) != 0 and b"run out of range" in msg.value
 print("host pack checks passed")
+# neighbour-list binning: bins are consistent and cover every pair within rcut
+import nbr_oracle as NO  # noqa: E402
+
+i32p = C.POINTER(C.c_int32)
+
+
+def _grid(x, cell, pbc, rcut):
+    n = len(x)
+    pos = np.ascontiguousarray(np.asarray(x, dtype=np.float64).T)
+    nb = (C.c_int * 3)()
+    R = (C.c_int * 3)()
+    bin_ = np.zeros(max(n, 1), np.int32)
+    bstart = np.zeros(2 * n + 66, np.int32)
+    batoms = np.zeros(max(n, 1), np.int32)
+    wrap = np.zeros(3 * max(n, 1), np.int32)
+    pw = np.zeros(3 * max(n, 1))
+    cellc = np.ascontiguousarray(cell, dtype=np.float64)
+    pbcc = np.asarray(pbc, dtype=np.int32)
+    m = C.create_string_buffer(256)
+    rc = lib.acehost_nbr_grid(C.c_int64(n), pos.ctypes.data_as(f64p), cellc.ctypes.data_as(f64p),
+                              pbcc.ctypes.data_as(i32p), C.c_double(rcut), nb, R, bin_.ctypes.data_as(i32p),
+                              bstart.ctypes.data_as(i32p), batoms.ctypes.data_as(i32p), wrap.ctypes.data_as(i32p),
+                              pw.ctypes.data_as(f64p), m, 256)
+    return rc, m.value, list(nb), list(R), bin_[:n], bstart, batoms[:n], wrap[:3 * n].reshape(3, n).T, \
+        pw[:3 * n].reshape(3, n).T
+
+
+lib.acehost_nbr_grid.argtypes = [C.c_int64, f64p, f64p, i32p, C.c_double, C.c_void_p, C.c_void_p, i32p, i32p,
+                                 i32p, i32p, f64p, C.c_char_p, C.c_int]
+tri = np.array([[4.0, 0.0, 0.0], [1.3, 3.7, 0.0], [0.6, -0.9, 4.4]])
+cases = [(tri, [1, 1, 1], 200, 3.1, True), (tri, [1, 1, 0], 150, 2.5, True), (20 * np.eye(3), [0, 0, 0], 60, 4.0, False),
+         (NO.fcc_cell(3.6), [1, 1, 1], 1, 5.0, True), (np.diag([3.0, 3.0, 30.0]), [1, 1, 0], 5, 7.5, True),
+         (tri * 3, [1, 1, 1], 500, 2.2, False)]
+for cell, pbc, n, rcut, outside in cases:
+    x = NO.random_system(n, cell, pbc, seed=n, outside=outside)
+    rc, m, nb, R, bin_, bstart, batoms, wrap, pw = _grid(x, cell, pbc, rcut)
+    assert rc == 0, m
+    nbins = nb[0] * nb[1] * nb[2]
+    assert nbins <= 2 * n + 64
+    assert bstart[0] == 0 and bstart[nbins] == n and np.all(np.diff(bstart[:nbins + 1]) >= 0)
+    assert sorted(batoms.tolist()) == list(range(n))
+    for b in range(nbins):
+        assert np.all(bin_[batoms[bstart[b]:bstart[b + 1]]] == b)
+    assert np.allclose(pw, x - wrap @ cell, atol=1e-12)
+    sw = pw @ np.linalg.inv(cell)
+    per = np.asarray(pbc, bool)
+    assert np.all(sw[:, per] >= -1e-12) and np.all(sw[:, per] < 1 + 1e-12)
+    i, j, S, r = NO.neighbour_pairs(x, cell, pbc, rcut)
+    Sw = S + wrap[j] - wrap[i]                       # shift between wrapped positions
+    nbv = np.asarray(nb)
+    bc = np.stack([bin_ % nb[0], (bin_ // nb[0]) % nb[1], bin_ // (nb[0] * nb[1])], -1)
+    reach = bc[j] + Sw * nbv - bc[i]
+    assert np.all(np.abs(reach) <= np.asarray(R)), (cell, pbc, nb, R)
+    assert np.all(Sw[:, ~per] == 0)
+for cell, pbc, rcut, x, expect in [(np.zeros((3, 3)), [1, 1, 1], 1.0, np.zeros((2, 3)), b"singular"),
+                                   (np.eye(3), [1, 1, 1], -1.0, np.zeros((2, 3)), b"rcut"),
+                                   (np.eye(3), [1, 2, 1], 1.0, np.zeros((2, 3)), b"pbc"),
+                                   (np.eye(3), [1, 1, 1], 1.0, np.array([[0.0, np.nan, 0.0]]), b"finite"),
+                                   (0.01 * np.eye(3), [1, 1, 1], 5.0, np.zeros((1, 3)), b"too small")]:
+    rc, m, *_ = _grid(x, cell, pbc, rcut)
+    assert rc != 0 and expect in m, (expect, m)
+print("host neighbour-grid checks passed")
+

@@ -41,6 +41,7 @@ def test_host_pack_logic_under_asan(tmp_path):
                        env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert "host pack checks passed" in r.stdout
+    assert "host neighbour-grid checks passed" in r.stdout
 
 
 def test_cpu_port_matches_oracle():
