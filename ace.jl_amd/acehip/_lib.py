@@ -51,7 +51,8 @@ EXPORTS = ("ace_ctx_create", "ace_set_params", "ace_set_tile", "ace_set_parts", 
            "ace_basis_dB", "ace_surr_features", "ace_surr_energy_forces",
            "ace_last_timing", "ace_set_validation", "ace_last_error",
            "ace_ctx_destroy", "ace_abi_version", "ace_nbrlist_create", "ace_nbrlist_destroy",
-           "ace_nbrlist_build", "ace_nbrlist_update", "ace_nbrlist_forces")
+           "ace_nbrlist_build", "ace_nbrlist_update", "ace_nbrlist_forces", "ace_nbrlist_forces_multi",
+           "ace_adjoint_EVAL_D")
 
 ERR_CAPACITY = -2  # ACE_ERR_CAPACITY
 
@@ -98,6 +99,8 @@ def load(path: str | None = None):
                                       vp, vp, C.POINTER(C.c_int64), vp]
     lib.ace_nbrlist_update.argtypes = [vp, C.c_int64, vp, _f64p, C.c_int64, vp, vp, vp, vp, vp]
     lib.ace_nbrlist_forces.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp]
+    lib.ace_nbrlist_forces_multi.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, C.c_int64, vp, vp, vp]
+    lib.ace_adjoint_EVAL_D.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
     if lib.ace_abi_version() != ABI_VERSION:
         raise AceError("libacehip.so ABI version mismatch")
     if path is None:

@@ -154,6 +154,17 @@ class NeighbourList:
                                                _vp(self.xyz), _vp(dE_dr), _vp(F), _vp(W), self._stream(stream)))
         return (F, W) if virial else F
 
+    def forces_multi(self, g, stream=None):
+        """Forces of many fields at once: g (nnbr, nvec, 3) -> F (natoms, nvec, 3)."""
+        if g.dim() != 3 or g.shape[0] != self.nnbr or g.shape[2] != 3:
+            raise ValueError("g must have shape (nnbr, nvec, 3) of this list")
+        g = g.contiguous()
+        F = torch.empty((self.natoms, g.shape[1], 3), dtype=torch.float64, device=self.dev)
+        _lib.check(self.lib.ace_nbrlist_forces_multi(self.h, self.natoms, _vp(self.offsets), _vp(self.rev),
+                                                     self.nnbr, g.shape[1], _vp(g), _vp(F),
+                                                     self._stream(stream)))
+        return F
+
 
 class ACECalculator:
     """Site-energy model on configurations: total energy, forces, virial.
@@ -208,6 +219,24 @@ class ACECalculator:
 
     def forces(self, at: Atoms):
         return self.energy_forces(at)[1]
+
+    # -- basis level: energy(basis, at) and forces(basis, at) of least-squares
+    #    fitting (src/testing/testlsq.jl:50-61)
+    def basis_energy(self, at: Atoms):
+        """sum_i B(env_i): (nB,) device tensor."""
+        nl = self._list(at)
+        return self.model.basis_B(nl.batch()).sum(0)
+
+    def basis_energy_forces(self, at: Atoms):
+        """(sum_i B(env_i) (nB,), per-basis-function forces (nB, N, 3)) as
+        device tensors; the forces are -d/dx of the summed basis."""
+        nl = self._list(at)
+        b = nl.batch()
+        EB = self.model.basis_B(b).sum(0)
+        if nl.nnbr == 0:
+            return EB, torch.zeros((len(EB), nl.natoms, 3), dtype=torch.float64, device=nl.dev)
+        dB = self.model.basis_dB(b)
+        return EB, nl.forces_multi(dB).permute(1, 0, 2)
 
     def virial(self, at: Atoms):
         return self.energy_forces(at, virial=True)[2]

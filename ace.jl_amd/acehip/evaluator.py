@@ -163,6 +163,17 @@ class HIPEvaluator:
                                          _vp(out), self._stream(stream)))
         return out
 
+    def adjoint_EVAL_D(self, b: EnvBatch, w, stream=None):
+        """dB[nenv, nB] = sum_j w_j . dB/dr_j for weights w (3, nnbr)
+        (evaluator.jl:204-244), Invariant bases."""
+        w = w.to(device=b.xyz.device, dtype=torch.float64).contiguous()
+        if tuple(w.shape) != (3, b.nnbr):
+            raise ValueError("w must have shape (3, nnbr)")
+        out = torch.empty((b.nenv, self.pack.nB), dtype=torch.float64, device=b.xyz.device)
+        _lib.check(self.lib.ace_adjoint_EVAL_D(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
+                                               _vp(w), _vp(out), self._stream(stream)))
+        return out
+
 
 # ---------------------------------------------------------------------------
 # ACE.jl-shaped functional API
@@ -224,6 +235,15 @@ def grad_config(model, cfg, species=None):
 def grad_params(model, cfg, species=None):
     """``grad_params`` = B (linearmodel.jl:114-123)."""
     return evaluate(model.basis, cfg, species)
+
+
+def adjoint_EVAL_D(model, cfg, w, species=None):
+    """``adjoint_EVAL_D(m, cfg, w)`` (linearmodel.jl:133-134 ->
+    evaluator.jl:204-244): sum_j w_j . dB/dr_j for one configuration (w: nX x 3)."""
+    b, single = _as_batch(cfg, species)
+    wt = torch.as_tensor(np.asarray(w, dtype=np.float64).reshape(-1, 3).T.copy())
+    out = _ev(model).adjoint_EVAL_D(b, wt).cpu().numpy()
+    return out[0] if single else out
 
 
 def evaluate_d(basis, cfg, species=None):

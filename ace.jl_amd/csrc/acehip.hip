@@ -870,6 +870,153 @@ __global__ __launch_bounds__(64) void k_dphi(RadialParams rp, int64_t nnbr,
     }
 }
 
+// k_poolw: dAw+[k][env] = sum_j w_j . grad_j phi_k over the pooled (m >= 0)
+// 1p functions: the directional derivative of A+ along per-neighbour weights
+// w[3][nnbr] (evaluator.jl:213-223, dAw = sum_j contract(w_j, dA[k, j]),
+// restated on A+; m < 0 follows by conjugation since w is real).  One wave
+// per environment; neighbours staged POOL_CH at a time in LDS as
+// [R_n | w.grad R_n | Y_lm | w.grad Y_lm | species] (radial and Ylm
+// gradients as in k_dphi), then lanes = pooled functions sum over them in
+// order: dAw+[k] += w.grad R_n Y_lm + R_n w.grad Y_lm.
+template <int L>
+__global__ __launch_bounds__(64) void k_poolw(RadialParams rp, int64_t nenv, const int32_t* __restrict__ off,
+                                              const double* __restrict__ xyz, const double* __restrict__ w,
+                                              int64_t nnbr, const int8_t* __restrict__ species,
+                                              const int32_t* __restrict__ pool, int npool,
+                                              double2* __restrict__ dAw, int64_t estride) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    const int64_t env = blockIdx.x;
+    const int nr = rp.nr;
+    const int rstride = 2 * nr + 4 * NP + 1;
+    if (env >= nenv) return;
+    const int j0 = off[env], j1 = off[env + 1];
+    int kr[POOL_KP], ky[POOL_KP], kmu[POOL_KP];
+    double ar[POOL_KP], ai[POOL_KP];
+#pragma unroll
+    for (int p = 0; p < POOL_KP; ++p) {
+        const int k = lane + 64 * p;
+        kr[p] = ky[p] = 0;
+        kmu[p] = -2;
+        if (k < npool) {
+            kmu[p] = pool[4 * k];
+            kr[p] = pool[4 * k + 1] - 1;
+            ky[p] = 2 * nr + 2 * idxp(pool[4 * k + 2], pool[4 * k + 3]);
+        }
+        ar[p] = ai[p] = 0.0;
+    }
+    for (int jb = j0; jb < j1; jb += POOL_CH) {
+        const int nch = min(POOL_CH, j1 - jb);
+        if (lane < nch) {
+            const int j = jb + lane;
+            const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+            const double wx = w[j], wy = w[nnbr + j], wz = w[2 * nnbr + j];
+            double* slot = smem + lane * rstride;
+            const Sph S = cart2spher(x, y, z);
+            double t, dtdr;
+            transform_ed(rp, S.r, t, dtdr);
+            const double wr = (wx * x + wy * y + wz * z) / S.r;  // w . rhat
+            // R_n and dR_n/dt by the recursion of k_dphi; slot holds R_n and
+            // w . grad R_n = dR_n/dt dt/dr (w . rhat)
+            double Rm2 = 0.0, dRm2 = 0.0;
+            double Rm1 = rp.A[0] * fcut(rp, t), dRm1 = rp.A[0] * fcut_d(rp, t);
+            slot[0] = Rm1;
+            slot[nr] = dRm1 * dtdr * wr;
+            for (int n = 1; n < nr; ++n) {
+                const double al = rp.A[n] * t + rp.B[n];
+                double Rn = al * Rm1, dRn = al * dRm1 + rp.A[n] * Rm1;
+                if (n >= 2) {
+                    Rn += rp.C[n] * Rm2;
+                    dRn += rp.C[n] * dRm2;
+                }
+                Rm2 = Rm1;
+                dRm2 = dRm1;
+                Rm1 = Rn;
+                dRm1 = dRn;
+                slot[n] = Rn;
+                slot[nr + n] = dRn * dtdr * wr;
+            }
+            double P[NP], dP[NP];
+            alp_eval_ed<L>(S, P, dP);
+            const double rinv = 1.0 / (S.r + 2.220446049250313e-16);
+            // w . grad = (fphi-part) + (ftheta-part) with the Cartesian rows of k_dphi
+            const double gph = (-S.sphi * wx + S.cphi * wy) * rinv;
+            const double gth = (S.cphi * S.cth * wx + S.sphi * S.cth * wy - S.sth * wz) * rinv;
+            double* ys = slot + 2 * nr;
+            const double isq2 = 0.7071067811865475;
+            for (int l = 0; l <= L; ++l) {
+                const int i = idxp(l, 0);
+                ys[2 * i] = P[i] * isq2;
+                ys[2 * i + 1] = 0.0;
+                ys[2 * NP + 2 * i] = gth * dP[i] * isq2;
+                ys[2 * NP + 2 * i + 1] = 0.0;
+            }
+            double epr = isq2, epi = 0.0;
+            for (int m = 1; m <= L; ++m) {
+                const double a = epr * S.cphi - epi * S.sphi;
+                const double b = epr * S.sphi + epi * S.cphi;
+                epr = a;
+                epi = b;
+                for (int l = m; l <= L; ++l) {
+                    const int i = idxp(l, m);
+                    const double pds = P[i], dpt = dP[i];
+                    ys[2 * i] = epr * pds * S.sth;
+                    ys[2 * i + 1] = epi * pds * S.sth;
+                    // fphi = i m ep P~, ftheta = ep dP~
+                    ys[2 * NP + 2 * i] = gph * (-m * epi * pds) + gth * (epr * dpt);
+                    ys[2 * NP + 2 * i + 1] = gph * (m * epr * pds) + gth * (epi * dpt);
+                }
+            }
+            slot[2 * nr + 4 * NP] = species ? (double)species[j] : -1.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int jj = 0; jj < nch; ++jj) {
+            const double* slot = smem + jj * rstride;
+            const int smu = species ? (int)slot[2 * nr + 4 * NP] : -1;
+#pragma unroll
+            for (int p = 0; p < POOL_KP; ++p) {
+                if (kmu[p] == -1 || kmu[p] == smu) {
+                    const double Rv = slot[kr[p]], wdR = slot[nr + kr[p]];
+                    const double Yr = slot[ky[p]], Yi = slot[ky[p] + 1];
+                    const double dYr = slot[2 * NP + ky[p]], dYi = slot[2 * NP + ky[p] + 1];
+                    ar[p] += wdR * Yr + Rv * dYr;
+                    ai[p] += wdR * Yi + Rv * dYi;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+#pragma unroll
+    for (int p = 0; p < POOL_KP; ++p)
+        if (kmu[p] != -2) dAw[(int64_t)(lane + 64 * p) * estride + env] = make_double2(ar[p], ai[p]);
+}
+
+// dAAw[env][iAA] = sum_t dAw_t prod_{s != t} A_s by a left-to-right dual
+// product (p, dp) <- (p a, dp a + p da) (evaluator.jl:225-235).
+__global__ void k_AAdir(int64_t nenv, int nA, int nAA, int maxord, const int32_t* __restrict__ orders,
+                        const int32_t* __restrict__ iAA2iA, const double2* __restrict__ A,
+                        const double2* __restrict__ dA, double2* __restrict__ dAA) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nenv * nAA) return;
+    const int64_t env = i / nAA;
+    const int r = (int)(i % nAA);
+    const double2* Ae = A + env * nA;
+    const double2* dAe = dA + env * nA;
+    double2 p = make_double2(1.0, 0.0), dp = make_double2(0.0, 0.0);
+    const int o = orders[r];
+    for (int t = 0; t < o; ++t) {
+        const int v = iAA2iA[r * maxord + t];
+        const double2 a = Ae[v], da = dAe[v];
+        const double2 x = cmul(dp, a), y = cmul(p, da);
+        dp = make_double2(x.x + y.x, x.y + y.y);
+        p = cmul(p, a);
+    }
+    dAA[i] = dp;
+}
+
 // dAA[j][iAA][3] = sum_a (prod_{t != a} A_t) dphi[j][v_a]  (pibasis.jl:402-432)
 __global__ void k_dAA(int64_t nnbr, int nA, int nAA, int maxord, const int32_t* __restrict__ nbr_env,
                       const int32_t* __restrict__ orders, const int32_t* __restrict__ iAA2iA,
@@ -1125,7 +1272,7 @@ struct ace_ctx {
     int last_parts = 1;
     // scratch
     DevBuf<double> Epart;
-    DevBuf<double2> Aplus, Gbuf, Afull, AA, dphi, dAA;
+    DevBuf<double2> Aplus, Gbuf, Afull, AA, dphi, dAA, Wplus, Wfull;
     DevBuf<int32_t> nbr_env;
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -2103,6 +2250,46 @@ extern "C" int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
     else
         hipLaunchKernelGGL(k_dB<1>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, nnbr, ctx->nB,
                            ctx->nAA, ctx->rowptr.p, ctx->cols.p, ctx->vals.p, ctx->dAA.p, dB_out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <int L>
+static void launch_poolw(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
+                         const double* w, int64_t nnbr, const int8_t* species, int64_t estride) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    const size_t lds = (size_t)POOL_CH * (2 * ctx->nr + 4 * NP + 1) * sizeof(double);
+    set_func_lds((const void*)k_poolw<L>, lds);
+    hipLaunchKernelGGL(k_poolw<L>, dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, w,
+                       nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->pool.p, ctx->npool, ctx->Wplus.p,
+                       estride);
+}
+
+extern "C" int ace_adjoint_EVAL_D(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                                  const int8_t* species, const double* w, double* dB_out, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
+    if (ctx->prop != ACE_PROP_INVARIANT) return set_err("ace_adjoint_EVAL_D: Invariant bases only");
+    if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
+    if (nenv == 0) return 0;
+    if (!dB_out || (nnbr > 0 && !w)) return set_err("ace_adjoint_EVAL_D: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
+    const int64_t estride = ((nenv + 63) / 64) * 64;
+    if (ctx->Wplus.ensure((size_t)ctx->npool * estride) || ctx->Wfull.ensure((size_t)nenv * ctx->nA) ||
+        ctx->AA.ensure((size_t)nenv * ctx->nAA))
+        return -1;
+    DISPATCH_L(ctx->L, launch_poolw, ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+    HIP_TRY(hipGetLastError());
+    const int64_t n = nenv * ctx->nA;
+    hipLaunchKernelGGL(k_expandA, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nenv, ctx->nA,
+                       ctx->aspec_fid.p, ctx->aspec_sgn.p, ctx->Wplus.p, estride, ctx->Wfull.p);
+    const int64_t n1 = nenv * ctx->nAA;
+    hipLaunchKernelGGL(k_AAdir, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, nenv, ctx->nA, ctx->nAA,
+                       ctx->maxorder, ctx->orders.p, ctx->iAA2iA.p, ctx->Afull.p, ctx->Wfull.p, ctx->AA.p);
+    const int64_t n2 = nenv * ctx->nB;
+    hipLaunchKernelGGL(k_B<0>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, nenv, ctx->nB, ctx->nAA,
+                       ctx->rowptr.p, ctx->cols.p, ctx->vals.p, ctx->AA.p, dB_out);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -11,6 +11,8 @@
 //                       list from new positions (a list built with a skin
 //                       stays valid for several MD steps); stream-ordered,
 //                       no host synchronization.
+//   ace_nbrlist_forces_multi  the same for many fields at once, e.g. the
+//                       per-basis-function forces of least-squares fitting.
 //   ace_nbrlist_forces  F_k = -dE/dx_k from the per-neighbour gradients
 //                       g_m = +dE_i/dr_m (the ACE.jl grad_config sign) and
 //                       the virial W = -sum_m g_m r_m^T, deterministically
@@ -254,6 +256,31 @@ __global__ __launch_bounds__(256) void k_nbr_forces(int natoms, const int32_t* _
     if (threadIdx.x < 9) vpart[(int64_t)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// F[k][v][d] = sum_{m in N(k)} (g[m][v][d] - g[rev(m)][v][d]) for nvec
+// fields at once (per-basis-function forces): thread per (atom, field),
+// consecutive threads read consecutive fields of one entry.
+__global__ __launch_bounds__(256) void k_nbr_forces_multi(int natoms, const int32_t* __restrict__ off,
+                                                          const int32_t* __restrict__ rev, int64_t nvec,
+                                                          const double* __restrict__ g, double* __restrict__ F) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (int64_t)natoms * nvec) return;
+    const int k = (int)(t / nvec);
+    const int64_t v = t - (int64_t)k * nvec;
+    double fx = 0, fy = 0, fz = 0;
+    const int m1 = off[k + 1];
+    for (int m = off[k]; m < m1; ++m) {
+        const double* a = g + ((int64_t)m * nvec + v) * 3;
+        const double* b = g + ((int64_t)rev[m] * nvec + v) * 3;
+        fx += a[0] - b[0];
+        fy += a[1] - b[1];
+        fz += a[2] - b[2];
+    }
+    double* o = F + t * 3;
+    o[0] = fx;
+    o[1] = fy;
+    o[2] = fz;
+}
+
 __global__ void k_nbr_vsum(int nblk, const double* __restrict__ vpart, double* __restrict__ W) {
     const int c = threadIdx.x;
     if (c >= 9) return;
@@ -433,5 +460,21 @@ extern "C" int ace_nbrlist_forces(ace_nbrlist* nl, int64_t natoms, const int32_t
         hipLaunchKernelGGL(k_nbr_vsum, dim3(1), dim3(64), 0, st, nblk, (const double*)nl->vpart.p, virial);
         HIP_TRY(hipGetLastError());
     }
+    return 0;
+}
+
+extern "C" int ace_nbrlist_forces_multi(ace_nbrlist* nl, int64_t natoms, const int32_t* env_offsets,
+                                        const int32_t* rev, int64_t nnbr, int64_t nvec, const double* g, double* F,
+                                        void* stream) {
+    if (!nl) return set_err("ace_nbrlist_forces_multi: null argument");
+    if (natoms < 0 || natoms >= ((int64_t)1 << 31) - 1 || nnbr < 0 || nnbr >= ((int64_t)1 << 31) || nvec < 0)
+        return set_err("ace_nbrlist_forces_multi: size out of range");
+    const int64_t n = natoms * nvec;
+    if (n == 0) return 0;
+    if (!env_offsets || !F || (nnbr > 0 && (!rev || !g))) return set_err("ace_nbrlist_forces_multi: null argument");
+    HIP_TRY(hipSetDevice(nl->device));
+    hipLaunchKernelGGL(k_nbr_forces_multi, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (int)natoms, env_offsets, rev, nvec, g, F);
+    HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -186,6 +186,15 @@ int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
                  const double* xyz, const int8_t* species, double* dB_out,
                  void* stream);
 
+/* adjoint_EVAL_D(m, ::ProductEvaluator, cfg, w) (evaluator.jl:204-244),
+ * batched over environments (Invariant bases): with per-neighbour weights
+ * w[3][nnbr] (SoA, device), dB_out[nenv][nB] = sum_j w_j . dB/dr_j, the
+ * derivative of the basis along w without forming dB (linearmodel.jl:133-134;
+ * the mixed second derivative behind force-matching gradients,
+ * linearmodel.jl:195-229). */
+int ace_adjoint_EVAL_D(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                       const int8_t* species, const double* w, double* dB_out, void* stream);
+
 /* Learned-surrogate support (no reference counterpart; for the surrogate
  * path of a LinearACEModel, see acemlp.h).
  *
@@ -285,6 +294,15 @@ int ace_nbrlist_update(ace_nbrlist* nl, int64_t natoms, const double* pos, const
 int ace_nbrlist_forces(ace_nbrlist* nl, int64_t natoms, const int32_t* env_offsets, const int32_t* rev,
                        int64_t nnbr, const double* xyz, const double* dE_dr, double* F, double* virial,
                        void* stream);
+
+/* The same assembly for nvec fields at once: g[nnbr][nvec][3] (e.g. dB of
+ * ace_basis_dB, one field per basis function) -> F[natoms][nvec][3] =
+ * -d/dx of the summed field; the per-basis-function forces
+ * ``forces(basis, at)`` of least-squares fitting (src/testing/testlsq.jl:
+ * 55-61).  Stream-ordered, allocates nothing, deterministic. */
+int ace_nbrlist_forces_multi(ace_nbrlist* nl, int64_t natoms, const int32_t* env_offsets,
+                             const int32_t* rev, int64_t nnbr, int64_t nvec, const double* g, double* F,
+                             void* stream);
 
 #ifdef __cplusplus
 }
