@@ -6,8 +6,10 @@ from .onep import (Categorical1pBasis, Product1pBasis, Rn1pBasis, Rn_basis,
 from .pibasis import NoConstant, PIBasis, PIBasisSpec, SimpleSparseBasis
 from .coupling import EuclideanVector, Invariant
 from .symmbasis import LinearACEModel, SymmetricBasis, symmetric_basis
+from .fio import load_json, read_dict, save_json, write_dict
 
 __all__ = ["Categorical1pBasis", "Product1pBasis", "Rn1pBasis", "Rn_basis",
            "RnYlm_1pbasis", "Ylm1pBasis", "init1pspec", "NoConstant", "PIBasis",
            "PIBasisSpec", "SimpleSparseBasis", "EuclideanVector", "Invariant",
-           "LinearACEModel", "SymmetricBasis", "symmetric_basis"]
+           "LinearACEModel", "SymmetricBasis", "symmetric_basis", "load_json", "read_dict",
+           "save_json", "write_dict"]
