@@ -1017,6 +1017,112 @@ __global__ void k_AAdir(int64_t nenv, int nA, int nAA, int maxord, const int32_t
     dAA[i] = dp;
 }
 
+// One A2B nonzero of the fused basis-value kernel, pre-decoded on the host:
+// the AA column's order and pooled factor ids (k | ACE_CONJ), and the A2B
+// value times the sign of the m < 0 factors (Invariant; EuclideanVector
+// rows read their 3 complex values from the CSR array).
+struct BtRec {
+    int32_t o;
+    int32_t f[4];
+    int32_t z;  // CSR index of the nonzero
+    double coef;
+};
+static_assert(sizeof(BtRec) == 32, "BtRec is 8 dwords");
+
+// Product (DIR = false) or dual-number derivative (DIR = true) of the AA
+// column of record r, left to right, from the staged tiles.
+template <bool DIR, int ENVS>
+__device__ __forceinline__ double2 bt_product(const double2* As, const double2* Ws, const BtRec& r, int e) {
+    double2 a[4], da[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q < r.o) {
+            a[q] = lds_factor<ENVS>(As, r.f[q], e);
+            if (DIR) da[q] = lds_factor<ENVS>(Ws, r.f[q], e);
+        }
+    double2 p = make_double2(1.0, 0.0), dp = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q < r.o) {
+            if (DIR) {
+                const double2 x = cmul(dp, a[q]), y = cmul(p, da[q]);
+                dp = make_double2(x.x + y.x, x.y + y.y);
+            }
+            p = cmul(p, a[q]);
+        }
+    return DIR ? dp : p;
+}
+
+// k_Bt: basis values B = A2B . AA (DIR = false; pibasis.jl:265-275,
+// symmbasis.jl:248-264) or their directional derivative A2B . dAA along
+// per-neighbour weights (DIR = true; evaluator.jl:225-240) for a tile of
+// ENVS environments, without materialising AA.  The pooled A+ tile (and
+// dA+w for DIR) is staged in LDS as in k_dagr; every A2B nonzero (one BtRec,
+// read with one scalar load) recomputes its AA product from it.  A
+// workgroup is (tile, chunk of rows); each wave takes rpw consecutive rows
+// (with ENVS = 32 the two half-waves take alternate rows).  Workgroups of
+// one tile are consecutive on one XCD so the staged tile is read from that
+// XCD's L2.
+constexpr int BT_NW = 16;  // waves per k_Bt workgroup
+
+template <int PROP, bool DIR, int ENVS>
+__global__ __launch_bounds__(64 * BT_NW) void k_Bt(int64_t nenv, int npool, int nB, int nchunk, int rpw,
+                                            const int32_t* __restrict__ rowptr, const BtRec* __restrict__ recs,
+                                            const double* __restrict__ vals, const double2* __restrict__ Aplus,
+                                            const double2* __restrict__ Wplus, int64_t estride,
+                                            double* __restrict__ out) {
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    extern __shared__ double2 tilebuf[];
+    double2* As = tilebuf;
+    double2* Ws = tilebuf + (size_t)npool * ENVS;
+    const int64_t ntile = (nenv + ENVS - 1) / ENVS;
+    const int64_t lin = xcd_env(blockIdx.x, ntile * nchunk);
+    const int64_t t = lin / nchunk;
+    const int chunk = (int)(lin % nchunk);
+    const int64_t e0 = t * ENVS;
+    for (int i = threadIdx.x; i < npool * ENVS; i += 64 * BT_NW) {
+        const int k = i / ENVS;
+        const int64_t env = e0 + (i % ENVS);
+        double2 v = make_double2(0.0, 0.0), w = make_double2(0.0, 0.0);
+        if (env < nenv) {
+            v = Aplus[(int64_t)k * estride + env];
+            if (DIR) w = Wplus[(int64_t)k * estride + env];
+        }
+        As[i] = v;
+        if (DIR) Ws[i] = w;
+    }
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int e = lane % ENVS;
+    const int half = ENVS == 32 ? (lane >> 5) : 0;
+    const int64_t env = e0 + e;
+    const int b0 = (chunk * BT_NW + wave) * rpw;
+    const int b1 = min(b0 + rpw, nB);
+    for (int b = b0 + half; b < b1; b += (ENVS == 32 ? 2 : 1)) {
+        double s[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) s[q] = 0.0;
+        const int z1 = rowptr[b + 1];
+        for (int z = rowptr[b]; z < z1; ++z) {
+            const BtRec r = recs[z];
+            const double2 v = bt_product<DIR, ENVS>(As, Ws, r, e);
+            if (PROP == 0) {
+                s[0] += r.coef * v.x;
+            } else {
+                const double sg = r.coef;
+#pragma unroll
+                for (int q = 0; q < NC; ++q)
+                    s[q] += sg * (vals[6 * r.z + 2 * q] * v.x - vals[6 * r.z + 2 * q + 1] * v.y);
+            }
+        }
+        if (env < nenv) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) out[(env * nB + b) * NC + q] = s[q];
+        }
+    }
+}
+
 // dAA[j][iAA][3] = sum_a (prod_{t != a} A_t) dphi[j][v_a]  (pibasis.jl:402-432)
 __global__ void k_dAA(int64_t nnbr, int nA, int nAA, int maxord, const int32_t* __restrict__ nbr_env,
                       const int32_t* __restrict__ orders, const int32_t* __restrict__ iAA2iA,
@@ -1244,6 +1350,7 @@ struct ace_ctx {
     DevBuf<double> rA, rB, rC;
     DevBuf<int32_t> pool, aspec_fid, aspec4, orders, iAA2iA, rowptr, cols;
     DevBuf<double> aspec_sgn, vals;
+    DevBuf<BtRec> bt_rec;
     DevBuf<int16_t> kidx;
     int nrl[ACE_LMAX + 1] = {0};  // n bound per l over the pooled functions
     // flat force kernel (Invariant, one species, nr <= 16): slot table and the
@@ -1398,6 +1505,27 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         ctx->gslot_list.upload(pk->gslot_list, pk->gslot_ptr[pk->npool]))
         return fail("upload");
     {
+        // one record per A2B nonzero for the fused basis-value kernel: the AA
+        // column's pooled factor ids and order, and (Invariant) the A2B value
+        // times the product of the m < 0 factor signs (EuclideanVector: the sign)
+        std::vector<BtRec> rec(std::max(1, pk->a2b_nnz));
+        for (int z = 0; z < pk->a2b_nnz; ++z) {
+            const int col = pk->a2b_cols[z];
+            BtRec& r = rec[z];
+            r.o = pk->orders[col];
+            r.z = z;
+            double sg = 1.0;
+            for (int q = 0; q < 4; ++q) {
+                r.f[q] = 0;
+                if (q < r.o) {
+                    const int iA = pk->iAA2iA[(size_t)col * pk->maxorder + q];
+                    r.f[q] = pk->aspec_fid[iA];
+                    sg *= pk->aspec_sgn[iA];
+                }
+            }
+            r.coef = pk->prop == ACE_PROP_INVARIANT ? pk->a2b_vals[z] * sg : sg;
+        }
+        if (ctx->bt_rec.upload(rec.data(), rec.size())) return fail("upload");
         std::vector<int32_t> a4 = acehost::aspec_rows(pk);
         if (ctx->aspec4.upload(a4.data(), a4.size())) return fail("upload");
         std::vector<int16_t> kx = acehost::pool_lookup(pk);
@@ -2196,6 +2324,37 @@ extern "C" int ace_basis_AA(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
     return 0;
 }
 
+// Fused basis values (k_Bt): whether the pooled tile fits LDS, and the launch.
+static bool bt_fits(const ace_ctx* ctx, bool dir) {
+    const int envs = dir ? 32 : 64;
+    return (size_t)ctx->npool * envs * sizeof(double2) * (dir ? 2 : 1) <= 160 * 1024;
+}
+
+template <bool DIR>
+static int launch_bt(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estride, double* out) {
+    constexpr int ENVS = DIR ? 32 : 64;
+    const size_t lds = (size_t)ctx->npool * ENVS * sizeof(double2) * (DIR ? 2 : 1);
+    constexpr int NW = BT_NW;
+    const int rpw = std::max(1, (ctx->nB + NW * 12 - 1) / (NW * 12));
+    const int nchunk = (ctx->nB + NW * rpw - 1) / (NW * rpw);
+    const int64_t nblk = (nenv + ENVS - 1) / ENVS * nchunk;
+    if (nblk >= ((int64_t)1 << 31)) return set_err("batch too large for one launch");
+    if (ctx->prop == ACE_PROP_INVARIANT) {
+        set_func_lds((const void*)k_Bt<0, DIR, ENVS>, lds);
+        hipLaunchKernelGGL((k_Bt<0, DIR, ENVS>), dim3((unsigned)nblk), dim3(64 * NW), lds, st, nenv, ctx->npool,
+                           ctx->nB, nchunk, rpw, ctx->rowptr.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
+                           ctx->Wplus.p, estride, out);
+    } else {
+        if (DIR) return set_err("directional basis derivative: Invariant bases only");
+        set_func_lds((const void*)k_Bt<1, false, 64>, lds);
+        hipLaunchKernelGGL((k_Bt<1, false, 64>), dim3((unsigned)nblk), dim3(64 * NW), lds, st, nenv, ctx->npool,
+                           ctx->nB, nchunk, rpw, ctx->rowptr.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
+                           ctx->Wplus.p, estride, out);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 extern "C" int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
                            const int8_t* species, double* B_out, void* stream) {
     int64_t nnbr = 0;
@@ -2203,6 +2362,16 @@ extern "C" int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* off, const
     if (validate_inputs(ctx, nenv, off, nnbr, species)) return -1;
     if (nenv == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
+    {
+        const int64_t estride = ((nenv + 63) / 64) * 64;
+        const int fits = bt_fits(ctx, false);
+        if (fits) {
+            if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
+            DISPATCH_L(ctx->L, launch_pool, ctx, st, nenv, off, xyz, nnbr, species, estride);
+            HIP_TRY(hipGetLastError());
+            return launch_bt<false>(ctx, st, nenv, estride, B_out);
+        }
+    }
     if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
     if (basis_AA_internal(ctx, nenv, st)) return -1;
     const int64_t n = nenv * ctx->nB;
@@ -2274,8 +2443,16 @@ extern "C" int ace_adjoint_EVAL_D(ace_ctx* ctx, int64_t nenv, const int32_t* off
     if (nenv == 0) return 0;
     if (!dB_out || (nnbr > 0 && !w)) return set_err("ace_adjoint_EVAL_D: null argument");
     hipStream_t st = (hipStream_t)stream;
-    if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
     const int64_t estride = ((nenv + 63) / 64) * 64;
+    if (bt_fits(ctx, true)) {
+        if (ctx->Aplus.ensure((size_t)ctx->npool * estride) || ctx->Wplus.ensure((size_t)ctx->npool * estride))
+            return -1;
+        DISPATCH_L(ctx->L, launch_pool, ctx, st, nenv, off, xyz, nnbr, species, estride);
+        DISPATCH_L(ctx->L, launch_poolw, ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        HIP_TRY(hipGetLastError());
+        return launch_bt<true>(ctx, st, nenv, estride, dB_out);
+    }
+    if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
     if (ctx->Wplus.ensure((size_t)ctx->npool * estride) || ctx->Wfull.ensure((size_t)nenv * ctx->nA) ||
         ctx->AA.ensure((size_t)nenv * ctx->nAA))
         return -1;
