@@ -183,7 +183,7 @@ __device__ __forceinline__ int64_t xcd_env(int64_t b, int64_t n) {
 // turn into pooled functions k = lane + 64 p (p < POOL_KP, npool <= 256)
 // and sum over the staged neighbours in order, holding the sums in registers
 // across chunks.
-constexpr int POOL_CH = 32;
+constexpr int POOL_CH = 16;
 constexpr int POOL_KP = 4;
 
 template <int L>
