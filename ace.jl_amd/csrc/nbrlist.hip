@@ -161,28 +161,6 @@ __global__ __launch_bounds__(256) void k_nbr_search(NbrArgs a, int32_t* __restri
     if (FILL && m != mend) atomicOr(flag, 1);
 }
 
-// rev[m] = index of (j -> i, -S') in the list of j; flag bit 2 if absent.
-__global__ __launch_bounds__(256) void k_nbr_rev(int natoms, const int32_t* __restrict__ off,
-                                                 const int32_t* __restrict__ nbr_j, const int32_t* __restrict__ shift,
-                                                 int64_t nnbr, int32_t* __restrict__ rev, int32_t* __restrict__ flag) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= natoms) return;
-    const int m1 = off[i + 1];
-    for (int m = off[i]; m < m1; ++m) {
-        const int j = nbr_j[m];
-        const int s0 = -shift[m], s1 = -shift[nnbr + m], s2 = -shift[2 * nnbr + m];
-        int found = -1;
-        const int q1 = off[j + 1];
-        for (int q = off[j]; q < q1; ++q)
-            if (nbr_j[q] == i && shift[q] == s0 && shift[nnbr + q] == s1 && shift[2 * nnbr + q] == s2) {
-                found = q;
-                break;
-            }
-        rev[m] = found;
-        if (found < 0) atomicOr(flag, 2);
-    }
-}
-
 // Centre of entry m: the atom i with off[i] <= m < off[i+1] (off ascending).
 __device__ __forceinline__ int owner(const int32_t* off, int natoms, int m) {
     int lo = 0, hi = natoms;  // off[lo] <= m < off[hi]
@@ -192,6 +170,27 @@ __device__ __forceinline__ int owner(const int32_t* off, int natoms, int m) {
         else hi = mid;
     }
     return lo;
+}
+
+// rev[m] = index of (j -> i, -S') in the list of j, one thread per entry;
+// flag bit 2 if absent.
+__global__ __launch_bounds__(256) void k_nbr_rev(int natoms, const int32_t* __restrict__ off,
+                                                 const int32_t* __restrict__ nbr_j, const int32_t* __restrict__ shift,
+                                                 int64_t nnbr, int32_t* __restrict__ rev, int32_t* __restrict__ flag) {
+    const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (m >= nnbr) return;
+    const int i = owner(off, natoms, (int)m);
+    const int j = nbr_j[m];
+    const int s0 = -shift[m], s1 = -shift[nnbr + m], s2 = -shift[2 * nnbr + m];
+    int found = -1;
+    const int q1 = off[j + 1];
+    for (int q = off[j]; q < q1; ++q)
+        if (nbr_j[q] == i && shift[q] == s0 && shift[nnbr + q] == s1 && shift[2 * nnbr + q] == s2) {
+            found = q;
+            break;
+        }
+    rev[m] = found;
+    if (found < 0) atomicOr(flag, 2);
 }
 
 __global__ __launch_bounds__(256) void k_nbr_update(int natoms, NbrArgs a, const double* __restrict__ pos,
@@ -210,24 +209,39 @@ __global__ __launch_bounds__(256) void k_nbr_update(int natoms, NbrArgs a, const
     xyz[2 * nnbr + m] = (pos[2 * n + j] - pos[2 * n + i]) + uz;
 }
 
-// F_k = sum_{m in N(k)} (g_m - g_rev(m)); per-block virial partials
-// -sum g_m r_m^T reduced in a fixed order.
+// F_k = sum_{m in N(k)} (g_m - g_rev(m)) for FA_ATOMS atoms per block.  The
+// block's entries [off[a0], off[a0 + FA_ATOMS]) are read coalesced by all
+// threads, FA_CH at a time, as d_m = g_m - g_rev(m) into LDS; then one thread
+// per atom sums its own entries in order.  Virial partials -g_m r_m^T are
+// summed per thread over its strided entries and reduced in a fixed order.
+constexpr int FA_ATOMS = 64;
+constexpr int FA_CH = 1024;
+
 __global__ __launch_bounds__(256) void k_nbr_forces(int natoms, const int32_t* __restrict__ off,
                                                     const int32_t* __restrict__ rev, int64_t nnbr,
                                                     const double* __restrict__ g, const double* __restrict__ xyz,
                                                     double* __restrict__ F, double* __restrict__ vpart) {
-    __shared__ double red[9][256];
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    static_assert(9 * 256 <= 3 * FA_CH, "the virial reduction reuses the staging array");
+    __shared__ double dl[3][FA_CH];
+    double (*red)[256] = reinterpret_cast<double (*)[256]>(&dl[0][0]);
+    const int a0 = blockIdx.x * FA_ATOMS;
+    const int a1 = min(a0 + FA_ATOMS, natoms);
+    const int tid = threadIdx.x;
+    const int m0 = off[a0], m1 = off[a1];
+    // this thread's atom (tid < FA_ATOMS) and its entry range
+    const int k = a0 + tid;
+    const bool own = tid < FA_ATOMS && k < a1;
+    const int mk0 = own ? off[k] : 0, mk1 = own ? off[k + 1] : 0;
+    double fx = 0, fy = 0, fz = 0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (k < natoms) {
-        double fx = 0, fy = 0, fz = 0;
-        const int m1 = off[k + 1];
-        for (int m = off[k]; m < m1; ++m) {
+    for (int c0 = m0; c0 < m1; c0 += FA_CH) {
+        const int c1 = min(c0 + FA_CH, m1);
+        for (int m = c0 + tid; m < c1; m += 256) {
             const int r = rev[m];
             const double gx = g[m], gy = g[nnbr + m], gz = g[2 * nnbr + m];
-            fx += gx - g[r];
-            fy += gy - g[nnbr + r];
-            fz += gz - g[2 * nnbr + r];
+            dl[0][m - c0] = gx - g[r];
+            dl[1][m - c0] = gy - g[nnbr + r];
+            dl[2][m - c0] = gz - g[2 * nnbr + r];
             if (vpart) {
                 const double rx = xyz[m], ry = xyz[nnbr + m], rz = xyz[2 * nnbr + m];
                 v[0] -= gx * rx;
@@ -241,19 +255,31 @@ __global__ __launch_bounds__(256) void k_nbr_forces(int natoms, const int32_t* _
                 v[8] -= gz * rz;
             }
         }
+        __syncthreads();
+        if (own) {
+            const int s0 = max(mk0, c0), s1 = min(mk1, c1);
+            for (int m = s0; m < s1; ++m) {
+                fx += dl[0][m - c0];
+                fy += dl[1][m - c0];
+                fz += dl[2][m - c0];
+            }
+        }
+        __syncthreads();
+    }
+    if (own) {
         F[k] = fx;
         F[natoms + k] = fy;
         F[2 * natoms + k] = fz;
     }
     if (!vpart) return;  // uniform over the block
-    for (int c = 0; c < 9; ++c) red[c][threadIdx.x] = v[c];
+    for (int c = 0; c < 9; ++c) red[c][tid] = v[c];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int c = 0; c < 9; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        if (tid < s)
+            for (int c = 0; c < 9; ++c) red[c][tid] += red[c][tid + s];
         __syncthreads();
     }
-    if (threadIdx.x < 9) vpart[(int64_t)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
+    if (tid < 9) vpart[(int64_t)blockIdx.x * 9 + tid] = red[tid][0];
 }
 
 // F[k][v][d] = sum_{m in N(k)} (g[m][v][d] - g[rev(m)][v][d]) for nvec
@@ -281,12 +307,21 @@ __global__ __launch_bounds__(256) void k_nbr_forces_multi(int natoms, const int3
     o[2] = fz;
 }
 
-__global__ void k_nbr_vsum(int nblk, const double* __restrict__ vpart, double* __restrict__ W) {
-    const int c = threadIdx.x;
-    if (c >= 9) return;
+// W[c] = sum over blocks of vpart[b][c] in a fixed order: 64 lanes per
+// component take every 64th block, then a tree over the lanes.
+__global__ __launch_bounds__(576) void k_nbr_vsum(int nblk, const double* __restrict__ vpart,
+                                                  double* __restrict__ W) {
+    __shared__ double part[9][64];
+    const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += vpart[(int64_t)b * 9 + c];
-    W[c] = s;
+    for (int b = l; b < nblk; b += 64) s += vpart[(int64_t)b * 9 + c];
+    part[c][l] = s;
+    __syncthreads();
+    for (int h = 32; h > 0; h >>= 1) {
+        if (l < h) part[c][l] += part[c][l + h];
+        __syncthreads();
+    }
+    if (l == 0) W[c] = part[c][0];
 }
 
 __global__ void k_nbr_zero(int32_t* p, int n) {
@@ -409,9 +444,10 @@ extern "C" int ace_nbrlist_build(ace_nbrlist* nl, int64_t natoms, const double* 
     hipLaunchKernelGGL(k_nbr_search<true>, dim3(nblk), dim3(256), 0, st, a, nl->cnt.p, (const int32_t*)env_offsets,
                        pos, total, nbr_j, shift, xyz, atom_species, nbr_species, nl->flag.p);
     HIP_TRY(hipGetLastError());
-    if (rev) {
-        hipLaunchKernelGGL(k_nbr_rev, dim3(nblk), dim3(256), 0, st, n, (const int32_t*)env_offsets,
-                           (const int32_t*)nbr_j, (const int32_t*)shift, total, rev, nl->flag.p);
+    if (rev && total > 0) {
+        hipLaunchKernelGGL(k_nbr_rev, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, n,
+                           (const int32_t*)env_offsets, (const int32_t*)nbr_j, (const int32_t*)shift, total, rev,
+                           nl->flag.p);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpyAsync(&nl->hflag, nl->flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -451,13 +487,18 @@ extern "C" int ace_nbrlist_forces(ace_nbrlist* nl, int64_t natoms, const int32_t
     HIP_TRY(hipSetDevice(nl->device));
     hipStream_t st = (hipStream_t)stream;
     const int n = (int)natoms;
-    const int nblk = std::max(1, (n + 255) / 256);
+    if (n == 0) {
+        if (virial) hipLaunchKernelGGL(k_nbr_vsum, dim3(1), dim3(576), 0, st, 0, (const double*)nullptr, virial);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
+    const int nblk = (n + FA_ATOMS - 1) / FA_ATOMS;
     if (virial && nl->vpart.ensure((size_t)nblk * 9)) return -1;
     hipLaunchKernelGGL(k_nbr_forces, dim3((unsigned)nblk), dim3(256), 0, st, n, env_offsets, rev, nnbr, dE_dr, xyz, F,
                        virial ? nl->vpart.p : nullptr);
     HIP_TRY(hipGetLastError());
     if (virial) {
-        hipLaunchKernelGGL(k_nbr_vsum, dim3(1), dim3(64), 0, st, nblk, (const double*)nl->vpart.p, virial);
+        hipLaunchKernelGGL(k_nbr_vsum, dim3(1), dim3(576), 0, st, nblk, (const double*)nl->vpart.p, virial);
         HIP_TRY(hipGetLastError());
     }
     return 0;
