@@ -1333,9 +1333,13 @@ struct WaveCfg {
 };
 
 constexpr int FORCE_NRMAX = 16;  // radial functions held in registers by the force kernels
+constexpr int FF_QMAX = 256;  // (l, m >= 0, n) entries of the flat force kernel (npool <= 256)
+constexpr int FF_MAXE = 16;   // environments a 256-neighbour block stages in LDS
+
 struct ForceFlatTab {
     int8_t nrl[ACE_LMAX + 1];                                  // n bound per l
     int16_t slot[(ACE_LMAX + 1) * (ACE_LMAX + 2) / 2][FORCE_NRMAX];  // G slot of (l, m >= 0, n)
+    int16_t nq;  // number of (l, m, n) entries (their G slots in staging order: ace_ctx::ffq)
 };
 
 struct ace_ctx {
@@ -1351,6 +1355,7 @@ struct ace_ctx {
     DevBuf<int32_t> pool, aspec_fid, aspec4, orders, iAA2iA, rowptr, cols;
     DevBuf<double> aspec_sgn, vals;
     DevBuf<BtRec> bt_rec;
+    DevBuf<int32_t> ffq;  // G slots of the flat force kernel's staged (l, m, n) entries
     DevBuf<int16_t> kidx;
     int nrl[ACE_LMAX + 1] = {0};  // n bound per l over the pooled functions
     // flat force kernel (Invariant, one species, nr <= 16): slot table and the
@@ -1561,6 +1566,14 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
                     for (int n = 0; n < ctx->nrl[l]; ++n)
                         if (!seen[(size_t)idxp(l, m) * FORCE_NRMAX + n]) ctx->flat_force = false;
             if (pk->ngslots > 32767) ctx->flat_force = false;
+            // staging order of the flat force kernel: m, then l >= m, then n
+            std::vector<int32_t> qslot;
+            for (int m = 0; m <= pk->L && ctx->flat_force; ++m)
+                for (int l = m; l <= pk->L; ++l)
+                    for (int n = 0; n < ctx->nrl[l]; ++n) qslot.push_back(ctx->fflat.slot[idxp(l, m)][n]);
+            if ((int)qslot.size() > FF_QMAX) ctx->flat_force = false;
+            ctx->fflat.nq = (int16_t)std::min<size_t>(qslot.size(), FF_QMAX);
+            if (ctx->flat_force && ctx->ffq.upload(qslot.data(), qslot.size())) return fail("upload");
             ctx->nfold = (int)fold.size() / 2;
             if (ctx->flat_force && ctx->nfold > 0 && ctx->gfold.upload(fold.data(), fold.size()))
                 return fail("upload");
@@ -1920,17 +1933,12 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
 // every (l, m, n) in the kernel arguments.  Radial values stay in registers
 // and the ALP values are produced m-outer exactly as in k_force.
 
-template <int L, int NRM>
-__global__ __launch_bounds__(256) void k_forcef(RadialParams rp, ForceFlatTab ft, int64_t nnbr,
-                                                 const double* __restrict__ xyz,
-                                                 const int32_t* __restrict__ nbr_env,
-                                                 const double2* __restrict__ Gbuf, int64_t estride,
-                                                 double* __restrict__ dE) {
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= nnbr) return;  // no barriers below
+template <int L, int NRM, bool STAGED>
+__device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceFlatTab& ft, int64_t nnbr, int64_t j,
+                                            const double* __restrict__ xyz, const double2* __restrict__ Ge,
+                                            const double2* __restrict__ Gs, int64_t estride,
+                                            double* __restrict__ dE) {
     const int nr = rp.nr;
-    const int64_t env = nbr_env[j];
-    const double2* Ge = Gbuf + env;
     const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
     const Sph S = cart2spher(x, y, z);
     double t, dtdr;
@@ -1967,6 +1975,7 @@ __global__ __launch_bounds__(256) void k_forcef(RadialParams rp, ForceFlatTab ft
     const double rx = x / S.r, ry = y / S.r, rz = z / S.r;
     const double st2 = S.sth * S.sth;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    int q = 0;  // running (l, m, n) index of the staged table
     double epr = 0.7071067811865475, epi = 0.0;
     double Pmm = 0.3989422804014327, dPmm = 0.0;
     const double temp0 = 0.3989422804014327;
@@ -2042,7 +2051,7 @@ __global__ __launch_bounds__(256) void k_forcef(RadialParams rp, ForceFlatTab ft
 #pragma unroll
             for (int n = 0; n < NRM; ++n) {
                 if (n < nb) {
-                    const double2 G = Ge[(int64_t)sl[n] * estride];
+                    const double2 G = STAGED ? Gs[(q + n) * FF_MAXE] : Ge[(int64_t)sl[n] * estride];
                     const double Rn = R[n], dRn = dR[n] * dtdr;
                     const double gy = G.x * Yr - G.y * Yi;
                     g0 += dRn * gy * rx + Rn * (G.x * dYr[0] - G.y * dYi[0]);
@@ -2050,11 +2059,41 @@ __global__ __launch_bounds__(256) void k_forcef(RadialParams rp, ForceFlatTab ft
                     g2 += dRn * gy * rz + Rn * (G.x * dYr[2] - G.y * dYi[2]);
                 }
             }
+            q += nb;
         }
     }
     dE[j] = g0;
     dE[nnbr + j] = g1;
     dE[2 * nnbr + j] = g2;
+}
+
+template <int L, int NRM>
+__global__ __launch_bounds__(256, 4) void k_forcef(RadialParams rp, ForceFlatTab ft, int64_t nnbr,
+                                                 const double* __restrict__ xyz,
+                                                 const int32_t* __restrict__ nbr_env,
+                                                 const int32_t* __restrict__ ffq,
+                                                 const double2* __restrict__ Gbuf, int64_t estride,
+                                                 double* __restrict__ dE) {
+    extern __shared__ double2 Gst[];  // [q][FF_MAXE]
+    const int64_t jb = (int64_t)blockIdx.x * 256;
+    const int64_t j = jb + threadIdx.x;
+    const int64_t jl = min(jb + 255, nnbr - 1);
+    // the block's environments (neighbours are grouped by environment)
+    const int64_t e0 = nbr_env[jb];
+    const int ne = (int)(nbr_env[jl] - e0 + 1);
+    if (ne <= FF_MAXE) {
+        for (int i = threadIdx.x; i < ft.nq * ne; i += 256) {
+            const int qq = i / ne, el = i - qq * ne;
+            Gst[qq * FF_MAXE + el] = Gbuf[(int64_t)ffq[qq] * estride + e0 + el];
+        }
+        __syncthreads();
+        if (j >= nnbr) return;  // no barriers below
+        const int el = (int)(nbr_env[j] - e0);
+        forcef_lane<L, NRM, true>(rp, ft, nnbr, j, xyz, nullptr, Gst + el, estride, dE);
+    } else {
+        if (j >= nnbr) return;
+        forcef_lane<L, NRM, false>(rp, ft, nnbr, j, xyz, Gbuf + nbr_env[j], nullptr, estride, dE);
+    }
 }
 
 // G[primary slot] += G[extra slot] for pooled functions whose gradient
@@ -2083,12 +2122,13 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
                                ctx->gfold.p, ctx->Gbuf.p, estride);
         // radial functions held in registers: 12 when they suffice (fewer
         // registers, more waves per SIMD), else 16
+        const size_t glds = (size_t)ctx->fflat.nq * FF_MAXE * sizeof(double2);
         if (ctx->nr <= 12)
-            hipLaunchKernelGGL((k_forcef<L, 12>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), 0, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->Gbuf.p, estride, dE);
+            hipLaunchKernelGGL((k_forcef<L, 12>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), glds, st,
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
         else
-            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), 0, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->Gbuf.p, estride, dE);
+            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), glds, st,
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
         return;
     }
     const int nsp = std::max(1, ctx->nspecies);
