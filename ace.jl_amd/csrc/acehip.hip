@@ -242,27 +242,45 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
                 p1 = p2;
                 slot[n] = p2;
             }
-            double P[NP];
-            alp_eval<L>(S, P);
+            // Y_lm = e^{i m phi}/sqrt2 P_lm, m-outer: P_mm from P_{m-1,m-1}, then
+            // the l recursion at fixed m (the operations of alp_eval, in an
+            // order that keeps two degrees live), written straight to LDS
             double* ys = slot + nr;
             const double isq2 = 0.7071067811865475;
-#pragma unroll
-            for (int l = 0; l <= L; ++l) {
-                ys[2 * idxp(l, 0)] = P[idxp(l, 0)] * isq2;
-                ys[2 * idxp(l, 0) + 1] = 0.0;
-            }
             double epr = isq2, epi = 0.0;
+            double Pmm = 0.3989422804014327;  // sqrt(1/(2 pi))
 #pragma unroll
-            for (int m = 1; m <= L; ++m) {
-                const double nr_ = epr * S.cphi - epi * S.sphi;
-                const double ni_ = epr * S.sphi + epi * S.cphi;
-                epr = nr_;
-                epi = ni_;
+            for (int m = 0; m <= L; ++m) {
+                if (m == 1) {
+                    Pmm = -1.224744871391589 * S.sth * Pmm;
+                } else if (m >= 2) {
+                    Pmm = -sqrt(1.0 + 0.5 / m) * S.sth * Pmm;
+                }
+                if (m > 0) {
+                    const double nr_ = epr * S.cphi - epi * S.sphi;
+                    const double ni_ = epr * S.sphi + epi * S.cphi;
+                    epr = nr_;
+                    epi = ni_;
+                }
+                double Pa = 0.0, Pb = Pmm;
 #pragma unroll
                 for (int l = m; l <= L; ++l) {
-                    const double p = P[idxp(l, m)];
-                    ys[2 * idxp(l, m)] = epr * p;
-                    ys[2 * idxp(l, m) + 1] = epi * p;
+                    double Pl;
+                    if (l == m) {
+                        Pl = Pmm;
+                    } else if (l == m + 1) {
+                        Pl = m == 0 ? S.cth * 1.7320508075688772 * 0.3989422804014327
+                                    : S.cth * sqrt(2.0 * m + 3.0) * Pmm;
+                    } else {
+                        const int i = idxp(l, m);
+                        Pl = c_alpA[i] * (S.cth * Pb + c_alpB[i] * Pa);
+                    }
+                    if (l > m) {
+                        Pa = Pb;
+                        Pb = Pl;
+                    }
+                    ys[2 * idxp(l, m)] = (m == 0 ? isq2 : epr) * Pl;
+                    ys[2 * idxp(l, m) + 1] = m == 0 ? 0.0 : epi * Pl;
                 }
             }
             slot[nr + 2 * NP] = species ? (double)species[j] : -1.0;
