@@ -91,7 +91,7 @@ class HIPEvaluator:
         self.waves = int(waves)
 
     def set_parts(self, parts: int = 0):
-        """Workgroups per product-DAG tile: 0 chooses per call (default), else 1, 2 or 4."""
+        """Workgroups per product-DAG tile: 0 chooses per call (default), else 1 to 4."""
         _lib.check(self.lib.ace_set_parts(self.ctx, int(parts)))
 
     def set_validation(self, on: bool):

@@ -1344,7 +1344,7 @@ __global__ __launch_bounds__(256) void k_feat_grad(FeatArgs a, int64_t nenv, con
 // ---------------------------------------------------------------------------
 // host side
 
-constexpr int MAX_PARTS = 4;  // workgroups per product-DAG tile: 1, 2 or 4
+constexpr int MAX_PARTS = 4;  // workgroups per product-DAG tile: 1 to 4
 
 struct WaveCfg {
     int W;
@@ -1616,7 +1616,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             return fail("upload");
         if (pk->prop == ACE_PROP_INVARIANT) {
             const std::vector<int> t_all = acehost::mode_tasks(ctx->host, true), t_e = acehost::mode_tasks(ctx->host, false);
-            for (int P = 2; P <= MAX_PARTS; P *= 2) {
+            for (int P = 2; P <= MAX_PARTS; ++P) {
                 std::vector<int32_t> ptr, list;
                 acehost::lpt_lists(ctx->host, t_all, P * t.W, ptr, list);
                 if (wc->pp_all[P].upload(ptr.data(), ptr.size()) || wc->pl_all[P].upload(list.data(), list.size()))
@@ -1653,7 +1653,7 @@ extern "C" int ace_set_params(ace_ctx* ctx, const double* c, int64_t nB) {
 
 extern "C" int ace_set_parts(ace_ctx* ctx, int parts) {
     if (!ctx) return set_err("ace_set_parts: null context");
-    if (parts != 0 && parts != 1 && parts != 2 && parts != 4) return set_err("ace_set_parts: parts must be 0, 1, 2 or 4");
+    if (parts < 0 || parts > MAX_PARTS) return set_err("ace_set_parts: parts must be 0 (automatic) or 1-4");
     ctx->parts = parts;
     return 0;
 }
@@ -2234,17 +2234,17 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     if (ctx->prop == ACE_PROP_INVARIANT) {
         // Parts per tile: the grid runs in rounds of ncu workgroups, and the last
         // round may be nearly empty; splitting each tile's tasks over P
-        // workgroups shortens the rounds.  Pick P in {1, 2, 4} minimising
+        // workgroups shortens the rounds.  Pick P in 1..4 minimising
         // rounds(P) / P, with 2 % per extra part for re-staging the A+ tile.
         // A split is only considered while every virtual wave keeps at least 4
         // tasks (fewer leave the part workgroups unbalanced or idle).
         int P = ctx->parts;
         const int ntask_mode = forces ? wc->ptr_all[wc->W] : wc->ptr_e[wc->W];
         int pmax = 1;
-        while (pmax < MAX_PARTS && ntask_mode >= 4 * (2 * pmax) * wc->W) pmax *= 2;
+        while (pmax < MAX_PARTS && ntask_mode >= 4 * (pmax + 1) * wc->W) ++pmax;
         if (P == 0) {
             double best = 1e300;
-            for (int q = 1; q <= pmax; q *= 2) {
+            for (int q = 1; q <= pmax; ++q) {
                 const double rounds = (double)((ntile * q + ctx->ncu - 1) / ctx->ncu);
                 const double cost = rounds / q * (1.0 + 0.02 * (q - 1));
                 if (cost < best - 1e-12) {

@@ -146,7 +146,7 @@ int ace_set_tile(ace_ctx* ctx, int waves_per_tile);
 
 /* Workgroups per product-DAG tile (Invariant): each runs 1/parts of the
  * tile's tasks, which shortens the last, partly empty round of workgroups.
- * 0 (default) chooses 1, 2 or 4 per call from the tile and CU counts. */
+ * 0 (default) chooses 1 to 4 per call from the tile and CU counts. */
 int ace_set_parts(ace_ctx* ctx, int parts);
 
 /* Site energies, ProductEvaluator evaluate (evaluator.jl:121-147), batched.

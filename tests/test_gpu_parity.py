@@ -44,9 +44,9 @@ def test_config1_energy_forces(cuda, waves):
     assert rel_err(dE.T, g_ref) < F_TOL
 
 
-@pytest.mark.parametrize("parts", [1, 2, 4])
+@pytest.mark.parametrize("parts", [1, 2, 3, 4])
 def test_config2_tile_parts(cuda, parts):
-    """Each tile's tasks split over 1, 2 or 4 workgroups: energies summed from
+    """Each tile's tasks split over 1 to 4 workgroups: energies summed from
     the parts' shares, gradient tasks each in exactly one part."""
     print(f"[gpu] order-4 degree-12 basis, parts={parts}", flush=True)
     basis, T = basis_for(4, 12, 6)
@@ -62,8 +62,9 @@ def test_config2_tile_parts(cuda, parts):
 def test_set_parts_rejects_bad_values(cuda):
     basis, T = basis_for(3, 6, 4)
     ev = HIPEvaluator(basis, synth.coefficients(len(basis), 1), device=0)
-    with pytest.raises(Exception):
-        ev.set_parts(3)
+    for bad in (-1, 5):
+        with pytest.raises(Exception):
+            ev.set_parts(bad)
 
 
 def test_config1_energy_only(cuda):
