@@ -7,7 +7,7 @@
  * surrogate predicts the correlation-order-3/4 share of the LinearACEModel
  * site energy from the order <= 2 invariants (B functions of the same
  * SymmetricBasis, symmbasis.jl:297-315); plain ACE (libacehip.so) is the
- * fallback and the judge of its error.
+ * fallback and the reference its error is measured against.
  *
  * Inputs X are BF16 [nrows][160] (normalized features, zero padded), device
  * memory.  Parameters are FP32 masters [nmemb][acemlp_nparam()] with BF16
