@@ -30,6 +30,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/acehip.h"
 #include "host_util.hpp"
 #include "nbrlist_host.hpp"
@@ -330,16 +332,132 @@ __global__ void k_nbr_zero(int32_t* p, int n) {
 }
 
 // ---------------------------------------------------------------------------
+// binning on the device (the operations of nbrlist_host.hpp: nbr_frac,
+// nbr_bin1, wrapped positions), so that the bins match the host's exactly
+
+struct NbrFracArgs {
+    double cell[9], inv[9];
+    int pbc[3];
+    int natoms;
+};
+
+// Wrapped fractional coordinates sf[3][n], wrap counts, wrapped positions
+// pw[3][n], and per-block minima / maxima of sf (bmm[block][6]).  flag bit 4:
+// a non-finite position; bit 8: a wrap beyond 1e6 cells.
+__global__ __launch_bounds__(256) void k_nbr_frac(NbrFracArgs a, const double* __restrict__ pos,
+                                                  double* __restrict__ sf, int32_t* __restrict__ wrap,
+                                                  double* __restrict__ pw, double* __restrict__ bmm,
+                                                  int32_t* __restrict__ flag) {
+    __shared__ double red[6][256];
+    const int n = a.natoms;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    if (i < n) {
+        const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
+        int w[3] = {0, 0, 0};
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) atomicOr(flag, 4);
+        for (int d = 0; d < 3; ++d) {
+            double sv = x * a.inv[d] + y * a.inv[3 + d] + z * a.inv[6 + d];
+            if (a.pbc[d]) {
+                const double f = floor(sv);
+                if (!(fabs(f) <= 1e6)) {
+                    atomicOr(flag, 8);
+                } else {
+                    w[d] = (int)f;
+                }
+                sv -= f;
+                if (sv >= 1.0) sv = 0.0;
+            }
+            sf[d * n + i] = sv;
+            wrap[d * n + i] = w[d];
+            lo[d] = sv;
+            hi[d] = sv;
+        }
+        for (int q = 0; q < 3; ++q)
+            pw[q * n + i] = pos[q * n + i] - ((double)w[0] * a.cell[q] + (double)w[1] * a.cell[3 + q] +
+                                              (double)w[2] * a.cell[6 + q]);
+    }
+    for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = lo[d];
+        red[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int d = 0; d < 3; ++d) {
+                red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + h]);
+                red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + h]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) bmm[(int64_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// mm[0..2] = minima, mm[3..5] = maxima over the blocks (one block).
+__global__ __launch_bounds__(256) void k_nbr_minmax(int nblk, const double* __restrict__ bmm, double* __restrict__ mm) {
+    __shared__ double red[6][256];
+    double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int b = threadIdx.x; b < nblk; b += 256)
+        for (int c = 0; c < 6; ++c) v[c] = c < 3 ? fmin(v[c], bmm[(int64_t)b * 6 + c]) : fmax(v[c], bmm[(int64_t)b * 6 + c]);
+    for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int c = 0; c < 6; ++c)
+                red[c][threadIdx.x] = c < 3 ? fmin(red[c][threadIdx.x], red[c][threadIdx.x + h])
+                                            : fmax(red[c][threadIdx.x], red[c][threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) mm[threadIdx.x] = red[threadIdx.x][0];
+}
+
+struct NbrBinArgs {
+    int nb[3];
+    double s0[3], w[3];
+    int natoms;
+};
+
+__device__ __forceinline__ int bin1(const NbrBinArgs& g, double s, int d) {
+    const int v = (int)floor((s - g.s0[d]) / g.w[d]);
+    return min(max(v, 0), g.nb[d] - 1);
+}
+
+// bin[i] and the sort values (atom index)
+__global__ __launch_bounds__(256) void k_nbr_bin(NbrBinArgs g, const double* __restrict__ sf,
+                                                 int32_t* __restrict__ bin, int32_t* __restrict__ idx) {
+    const int n = g.natoms;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    bin[i] = (bin1(g, sf[2 * n + i], 2) * g.nb[1] + bin1(g, sf[n + i], 1)) * g.nb[0] + bin1(g, sf[i], 0);
+    idx[i] = i;
+}
+
+// bin_start[b] = first position of bin b in the sorted keys (b <= nbins)
+__global__ __launch_bounds__(256) void k_nbr_binstart(int nbins, int n, const int32_t* __restrict__ keys,
+                                                      int32_t* __restrict__ bin_start) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b > nbins) return;
+    int lo = 0, hi = n;  // first q with keys[q] >= b
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < b) lo = mid + 1;
+        else hi = mid;
+    }
+    bin_start[b] = lo;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 struct ace_nbrlist {
     int device = 0;
-    acehost::NbrGrid g;
-    std::vector<double> hpos;
+    acehost::NbrGrid g;  // grid parameters (nb, R, s0, w) of the last build
     std::vector<int32_t> hcnt, hoff;
     int32_t hflag = 0;
-    DevBuf<double> pw, vpart;
-    DevBuf<int32_t> wrap, bin, bin_start, bin_atoms, cnt, flag;
+    double hmm[6] = {0, 0, 0, 0, 0, 0};
+    DevBuf<double> pw, vpart, sf, bmm, mm;
+    DevBuf<int32_t> wrap, bin, bin_start, bin_atoms, cnt, flag, keys, idx;
+    DevBuf<char> sort_tmp;
 };
 
 static NbrArgs grid_args(ace_nbrlist* nl, const double* cell) {
@@ -352,7 +470,7 @@ static NbrArgs grid_args(ace_nbrlist* nl, const double* cell) {
         a.R[d] = g.R[d];
         a.pbc[d] = g.pbc[d];
     }
-    a.natoms = (int)(g.p.size() / 3);
+    a.natoms = (int)nl->hcnt.size();
     a.pw = nl->pw.p;
     a.wrap = nl->wrap.p;
     a.bin = nl->bin.p;
@@ -393,12 +511,8 @@ extern "C" int ace_nbrlist_build(ace_nbrlist* nl, int64_t natoms, const double* 
     HIP_TRY(hipSetDevice(nl->device));
     hipStream_t st = (hipStream_t)stream;
     const int n = (int)natoms;
-    nl->hpos.resize(3 * (size_t)n);
-    if (n > 0) {
-        HIP_TRY(hipMemcpyAsync(nl->hpos.data(), pos, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-    }
-    const std::string gerr = acehost::nbr_grid(natoms, nl->hpos.data(), cell, pbc, rcut, nl->g);
+    acehost::NbrGeom geo;
+    const std::string gerr = acehost::nbr_setup(cell, pbc, rcut, geo);
     if (!gerr.empty()) return set_err("ace_nbrlist_build: " + gerr);
     nl->hoff.assign((size_t)n + 1, 0);
     if (n == 0) {
@@ -406,24 +520,60 @@ extern "C" int ace_nbrlist_build(ace_nbrlist* nl, int64_t natoms, const double* 
         HIP_TRY(hipStreamSynchronize(st));
         return 0;
     }
-    const acehost::NbrGrid& g = nl->g;
-    if (nl->pw.ensure(3 * (size_t)n) || nl->wrap.ensure(3 * (size_t)n) || nl->bin.ensure(n) ||
-        nl->bin_start.ensure(g.bin_start.size()) || nl->bin_atoms.ensure(n) || nl->cnt.ensure(n) ||
-        nl->flag.ensure(1))
-        return -1;
-    HIP_TRY(hipMemcpyAsync(nl->pw.p, g.p.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(nl->wrap.p, g.wrap.data(), 3 * (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(nl->bin.p, g.bin.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(nl->bin_start.p, g.bin_start.data(), g.bin_start.size() * sizeof(int32_t),
-                           hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(nl->bin_atoms.p, g.bin_atoms.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
-                           st));
-    const NbrArgs a = grid_args(nl, cell);
     const unsigned nblk = (unsigned)((n + 255) / 256);
+    if (nl->pw.ensure(3 * (size_t)n) || nl->wrap.ensure(3 * (size_t)n) || nl->sf.ensure(3 * (size_t)n) ||
+        nl->bin.ensure(n) || nl->idx.ensure(n) || nl->keys.ensure(n) || nl->bin_atoms.ensure(n) ||
+        nl->cnt.ensure(n) || nl->flag.ensure(1) || nl->bmm.ensure((size_t)nblk * 6) || nl->mm.ensure(6))
+        return -1;
+    // wrap and bin on the device: fractional coordinates, extents (for open axes)
+    NbrFracArgs fa{};
+    for (int q = 0; q < 9; ++q) {
+        fa.cell[q] = geo.cell[q];
+        fa.inv[q] = geo.inv[q];
+    }
+    for (int d = 0; d < 3; ++d) fa.pbc[d] = geo.pbc[d];
+    fa.natoms = n;
+    hipLaunchKernelGGL(k_nbr_zero, dim3(1), dim3(64), 0, st, nl->flag.p, 1);
+    hipLaunchKernelGGL(k_nbr_frac, dim3(nblk), dim3(256), 0, st, fa, pos, nl->sf.p, nl->wrap.p, nl->pw.p, nl->bmm.p,
+                       nl->flag.p);
+    hipLaunchKernelGGL(k_nbr_minmax, dim3(1), dim3(256), 0, st, (int)nblk, (const double*)nl->bmm.p, nl->mm.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(nl->hmm, nl->mm.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&nl->hflag, nl->flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nl->hflag & 4) return set_err("ace_nbrlist_build: positions must be finite");
+    if (nl->hflag & 8) return set_err("ace_nbrlist_build: positions too far outside the cell");
+    acehost::NbrGrid& g = nl->g;
+    const std::string perr = acehost::nbr_params(geo, natoms, nl->hmm, nl->hmm + 3, g);
+    if (!perr.empty()) return set_err("ace_nbrlist_build: " + perr);
+    const int nbins = g.nbins();
+    if (nl->bin_start.ensure((size_t)nbins + 1)) return -1;
+    NbrBinArgs ba{};
+    for (int d = 0; d < 3; ++d) {
+        ba.nb[d] = g.nb[d];
+        ba.s0[d] = g.s0[d];
+        ba.w[d] = g.w[d];
+    }
+    ba.natoms = n;
+    hipLaunchKernelGGL(k_nbr_bin, dim3(nblk), dim3(256), 0, st, ba, (const double*)nl->sf.p, nl->bin.p, nl->idx.p);
+    HIP_TRY(hipGetLastError());
+    // stable sort of the atoms by bin (ascending atom index within a bin)
+    int end_bit = 1;
+    while (end_bit < 31 && (1 << end_bit) < nbins) ++end_bit;
+    size_t tmp_bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, nl->bin.p, nl->keys.p, nl->idx.p,
+                                               nl->bin_atoms.p, n, 0, end_bit, st));
+    if (nl->sort_tmp.ensure(std::max<size_t>(tmp_bytes, 1))) return -1;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nl->sort_tmp.p, tmp_bytes, nl->bin.p, nl->keys.p, nl->idx.p,
+                                               nl->bin_atoms.p, n, 0, end_bit, st));
+    hipLaunchKernelGGL(k_nbr_binstart, dim3((unsigned)((nbins + 1 + 255) / 256)), dim3(256), 0, st, nbins, n,
+                       (const int32_t*)nl->keys.p, nl->bin_start.p);
+    HIP_TRY(hipGetLastError());
+    nl->hcnt.resize(n);
+    const NbrArgs a = grid_args(nl, cell);
     hipLaunchKernelGGL(k_nbr_search<false>, dim3(nblk), dim3(256), 0, st, a, nl->cnt.p, nullptr, nullptr,
                        (int64_t)0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     HIP_TRY(hipGetLastError());
-    nl->hcnt.resize(n);
     HIP_TRY(hipMemcpyAsync(nl->hcnt.data(), nl->cnt.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     int64_t total = 0;

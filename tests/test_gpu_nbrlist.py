@@ -200,3 +200,21 @@ def test_skin_reuses_the_list(cuda):
     E1, _ = skin.energy_forces(at)
     assert skin.rebuilds == 2
     assert abs(E1 - plain.energy(at)) <= 1e-12 * abs(E1)
+
+
+def test_bad_positions_are_rejected(cuda):
+    print("[gpu] neighbour list: non-finite and far-away positions", flush=True)
+    from acehip._lib import AceError
+    x = N.random_system(20, TRI, [1, 1, 1], seed=4)
+    nl = NeighbourList(0)
+    bad = x.copy()
+    bad[7, 1] = np.nan
+    with pytest.raises(AceError, match="finite"):
+        nl.build(bad, TRI, [1, 1, 1], 3.0)
+    far = x.copy()
+    far[3] += 5e6 * TRI[0]
+    with pytest.raises(AceError, match="too far"):
+        nl.build(far, TRI, [1, 1, 1], 3.0)
+    nl.build(x, TRI, [1, 1, 1], 3.0)         # the handle stays usable
+    torch.cuda.synchronize()
+    assert nl.nnbr == len(N.neighbour_pairs(x, TRI, [1, 1, 1], 3.0)[0])
