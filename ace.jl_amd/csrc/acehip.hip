@@ -315,6 +315,7 @@ struct DagArgs {
     const int32_t* runs;       // [nruns][7]
     const int32_t* leaf_fid;   // [nleaves]
     const double* leaf_coef;   // [nleaves][ncoef]
+    const double4* vleaf;      // EuclideanVector: [nleaves][2] decoded leaves (factor id, then 3 complex coefs)
     const int32_t* tasks;      // [ntasks][3]
     const double* task_const;  // [ntasks][2 | 6]
     const int32_t* wtask_ptr;  // [W+1]
@@ -384,15 +385,22 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
             double2 S[NC];
 #pragma unroll
             for (int p = 0; p < NC; ++p) S[p] = make_double2(0.0, 0.0);
-            for (int i = 0; i < nl; ++i) {
-                const int f = a.leaf_fid[lo + i];
-                const double2 v = lds_factor<ENVS>(As, f, el);
-                if (PROP == 0) {
+            if (PROP == 0) {
+                for (int i = 0; i < nl; ++i) {
+                    const int f = a.leaf_fid[lo + i];
+                    const double2 v = lds_factor<ENVS>(As, f, el);
                     const double c = a.leaf_coef[lo + i];
                     S[0].x += c * v.x;
                     S[0].y += c * v.y;
-                } else {
-                    const double* c = a.leaf_coef + 6 * (int64_t)(lo + i);
+                }
+            } else {
+                // one 64-byte decoded leaf per iteration: {factor id, 0, c0, c1}, {c2, c3, c4, c5};
+                // unrolled by 2 so that consecutive leaf fetches overlap
+#pragma unroll 2
+                for (int i = 0; i < nl; ++i) {
+                    const double4 h = a.vleaf[2 * (int64_t)(lo + i)], q = a.vleaf[2 * (int64_t)(lo + i) + 1];
+                    const double2 v = lds_factor<ENVS>(As, (int)__double2loint(h.x), el);
+                    const double c[6] = {h.z, h.w, q.x, q.y, q.z, q.w};
 #pragma unroll
                     for (int p = 0; p < NC; ++p) {
                         S[p].x += c[2 * p] * v.x - c[2 * p + 1] * v.y;
@@ -1388,6 +1396,7 @@ struct ace_ctx {
     DevBuf<int32_t> gfold;
     DevBuf<int32_t> runs, leaf_fid, d_tasks, gslot_ptr, gslot_list;
     DevBuf<double> leaf_coef, task_const, e0v;
+    DevBuf<double4> vleaf;  // EuclideanVector decoded leaves
     DevBuf<uint32_t> recs;
     // surrogate feature tables
     int nfeat = 0;
@@ -1422,6 +1431,20 @@ struct ace_ctx {
 static int upload_coeffs(ace_ctx* ctx, const double* c) {
     acehost::Coeffs cf = acehost::coefficients(ctx->host, c);
     if (ctx->leaf_coef.upload(cf.leaf.data(), cf.leaf.size())) return -1;
+    if (ctx->prop != ACE_PROP_INVARIANT) {
+        // decoded leaves: {factor id (low word), 0, c0, c1}, {c2, c3, c4, c5}
+        const size_t nl = ctx->host.leaf_fid.size();
+        std::vector<double4> vl(std::max<size_t>(2 * nl, 2));
+        for (size_t i = 0; i < nl; ++i) {
+            const double* c = cf.leaf.data() + 6 * i;
+            uint64_t bits = (uint32_t)ctx->host.leaf_fid[i];
+            double fid;
+            std::memcpy(&fid, &bits, sizeof(fid));
+            vl[2 * i] = make_double4(fid, 0.0, c[0], c[1]);
+            vl[2 * i + 1] = make_double4(c[2], c[3], c[4], c[5]);
+        }
+        if (ctx->vleaf.upload(vl.data(), vl.size())) return -1;
+    }
     if (ctx->prop == ACE_PROP_INVARIANT) {
         std::vector<uint32_t> rec;
         const std::string rerr = acehost::records(ctx->host, cf, rec);
@@ -2221,6 +2244,7 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     a.runs = ctx->runs.p;
     a.leaf_fid = ctx->leaf_fid.p;
     a.leaf_coef = ctx->leaf_coef.p;
+    a.vleaf = ctx->vleaf.p;
     a.tasks = ctx->d_tasks.p;
     a.task_const = ctx->task_const.p;
     a.wtask_ptr = forces ? wc->d_ptr_all.p : wc->d_ptr_e.p;

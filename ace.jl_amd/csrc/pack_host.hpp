@@ -173,6 +173,7 @@ struct PackCopy {
     int prop = 0, nAA = 0, nB = 0, nleaves = 0, ntasks = 0, nrec = 0;
     std::vector<uint32_t> rec_hdr;
     std::vector<int32_t> rec_leaf;
+    std::vector<int32_t> leaf_fid;  // [nleaves] factor id (k | 0x8000 conjugated)
     std::vector<int32_t> tasks;  // [ntasks][5] (run_b, run_e, slot, rec_b, rec_e)
     std::vector<int32_t> rowptr, cols, leaf_term_ptr, task_term_ptr, term_row, term_conj, orders;
     std::vector<double> vals, term_scale;
@@ -191,6 +192,7 @@ inline PackCopy copy_pack(const ace_pack* pk) {
     p.cols.assign(pk->a2b_cols, pk->a2b_cols + pk->a2b_nnz);
     p.vals.assign(pk->a2b_vals, pk->a2b_vals + (size_t)pk->a2b_nnz * nc);
     p.leaf_term_ptr.assign(pk->leaf_term_ptr, pk->leaf_term_ptr + pk->nleaves + 1);
+    p.leaf_fid.assign(pk->leaf_fid, pk->leaf_fid + pk->nleaves);
     p.task_term_ptr.assign(pk->task_term_ptr, pk->task_term_ptr + pk->ntasks + 1);
     p.term_row.assign(pk->term_row, pk->term_row + pk->nterms);
     p.term_scale.assign(pk->term_scale, pk->term_scale + pk->nterms);
