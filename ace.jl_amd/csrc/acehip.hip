@@ -2260,12 +2260,12 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
         // round may be nearly empty; splitting each tile's tasks over P
         // workgroups shortens the rounds.  Pick P in 1..4 minimising
         // rounds(P) / P, with 2 % per extra part for re-staging the A+ tile.
-        // A split is only considered while every virtual wave keeps at least 4
+        // A split is only considered while every virtual wave keeps at least 3
         // tasks (fewer leave the part workgroups unbalanced or idle).
         int P = ctx->parts;
         const int ntask_mode = forces ? wc->ptr_all[wc->W] : wc->ptr_e[wc->W];
         int pmax = 1;
-        while (pmax < MAX_PARTS && ntask_mode >= 4 * (pmax + 1) * wc->W) ++pmax;
+        while (pmax < MAX_PARTS && ntask_mode >= 3 * (pmax + 1) * wc->W) ++pmax;
         if (P == 0) {
             double best = 1e300;
             for (int q = 1; q <= pmax; ++q) {
