@@ -2,7 +2,8 @@
 // Expansion with ACE.jl v0.12.46 semantics.  FP64 throughout.
 //
 // Pipeline of an energy (+ forces) call:
-//   k_pool   : one wave per environment, lanes = neighbours.  Radial basis
+//   k_pool   : one wave per environment (two, one per half-wave, for
+//              bases with at most 32 pooled values), lanes = neighbours.  Radial basis
 //              (orthpolys.jl:253-263 after distancetransforms.jl:73) and the
 //              m >= 0 complex spherical harmonics (sphericalharmonics.jl:
 //              43-51,167-197,379-403) per neighbour, staged in LDS, then
@@ -178,15 +179,16 @@ __device__ __forceinline__ int64_t xcd_env(int64_t b, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// k_pool: A+[k][env].  One wave per environment.  Neighbours are staged
-// POOL_CH at a time in LDS ([slot][R_n | Y_lm | species]); the lanes then
-// turn into pooled functions k = lane + 64 p (p < POOL_KP, npool <= 256)
-// and sum over the staged neighbours in order, holding the sums in registers
-// across chunks.
+// k_pool: A+[k][env].  EPW environments per wave, one per group of
+// LPE = 64 / EPW lanes (EPW = 2 when npool <= 32, so that small bases keep
+// the lanes busy).  A group stages its environment's neighbours POOL_CH at a
+// time in its own LDS region ([slot][R_n | Y_lm | species]); its lanes then
+// turn into pooled functions k = hl + LPE p (p < KP) and sum over the staged
+// neighbours in order, holding the sums in registers across chunks.
 constexpr int POOL_CH = 16;
 constexpr int POOL_KP = 4;
 
-template <int L>
+template <int L, int EPW>
 __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
                                               const int32_t* __restrict__ off,
                                               const double* __restrict__ xyz, int64_t nnbr,
@@ -195,19 +197,24 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
                                               double2* __restrict__ Aplus, int64_t estride,
                                               int32_t* __restrict__ nbr_env) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
-    extern __shared__ double smem[];  // [POOL_CH][rstride]
+    constexpr int LPE = 64 / EPW;                 // lanes per environment
+    constexpr int KP = EPW == 1 ? POOL_KP : 1;    // pooled functions per lane
+    extern __shared__ double smem[];  // [EPW][POOL_CH][rstride]
     const int lane = threadIdx.x;
-    const int64_t env = xcd_env(blockIdx.x, nenv);
+    const int hl = lane % LPE;
     const int nr = rp.nr;
     const int rstride = nr + 2 * NP + 1;  // doubles per neighbour slot
-    if (env >= nenv) return;
-    const int j0 = off[env], j1 = off[env + 1];
-    // this lane's pooled functions k = lane + 64 p: slot offsets of R_n and Y_lm
-    int kr[POOL_KP], ky[POOL_KP], kmu[POOL_KP];
-    double ar[POOL_KP], ai[POOL_KP];
+    const int64_t nblk = (nenv + EPW - 1) / EPW;
+    const int64_t env = xcd_env(blockIdx.x, nblk) * EPW + lane / LPE;
+    const bool valid = env < nenv;
+    const int j0 = valid ? off[env] : 0, j1 = valid ? off[env + 1] : 0;
+    double* const sgrp = smem + (lane / LPE) * POOL_CH * rstride;
+    // this lane's pooled functions k = hl + LPE p: slot offsets of R_n and Y_lm
+    int kr[KP], ky[KP], kmu[KP];
+    double ar[KP], ai[KP];
 #pragma unroll
-    for (int p = 0; p < POOL_KP; ++p) {
-        const int k = lane + 64 * p;
+    for (int p = 0; p < KP; ++p) {
+        const int k = hl + LPE * p;
         kr[p] = ky[p] = 0;
         kmu[p] = -2;  // -2: no pooled function in this pass
         if (k < npool) {
@@ -219,11 +226,11 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
     }
     for (int jb = j0; jb < j1; jb += POOL_CH) {
         const int nch = min(POOL_CH, j1 - jb);
-        if (lane < nch) {
-            const int j = jb + lane;
+        if (hl < nch) {
+            const int j = jb + hl;
             if (nbr_env) nbr_env[j] = (int32_t)env;  // neighbour -> environment, for the flat force kernel
             const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
-            double* slot = smem + lane * rstride;
+            double* slot = sgrp + hl * rstride;
             Sph S = cart2spher(x, y, z);
             // radial basis
             double u = (1.0 + rp.r0) / (1.0 + S.r);
@@ -289,10 +296,10 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // lanes over pooled functions, neighbours summed in order
         for (int jj = 0; jj < nch; ++jj) {
-            const double* slot = smem + jj * rstride;
+            const double* slot = sgrp + jj * rstride;
             const int smu = species ? (int)slot[nr + 2 * NP] : -1;
 #pragma unroll
-            for (int p = 0; p < POOL_KP; ++p) {
+            for (int p = 0; p < KP; ++p) {
                 if (kmu[p] == -1 || kmu[p] == smu) {
                     const double R = slot[kr[p]];
                     ar[p] += R * slot[ky[p]];
@@ -303,9 +310,10 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
+    if (!valid) return;
 #pragma unroll
-    for (int p = 0; p < POOL_KP; ++p)
-        if (kmu[p] != -2) Aplus[(int64_t)(lane + 64 * p) * estride + env] = make_double2(ar[p], ai[p]);
+    for (int p = 0; p < KP; ++p)
+        if (kmu[p] != -2) Aplus[(int64_t)(hl + LPE * p) * estride + env] = make_double2(ar[p], ai[p]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1774,16 +1782,25 @@ template <int L>
 static void launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
                         const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
-    const size_t lds = (size_t)POOL_CH * (ctx->nr + 2 * NP + 1) * sizeof(double);
-    set_func_lds((const void*)k_pool<L>, lds);
+    // two environments per wave when a half-wave covers the pooled functions
+    const int epw = ctx->npool <= 32 ? 2 : 1;
+    const size_t lds = (size_t)epw * POOL_CH * (ctx->nr + 2 * NP + 1) * sizeof(double);
     int32_t* nbr_env = nullptr;
     if (ctx->flat_force && nnbr > 0) {
         if (ctx->nbr_env.ensure(nnbr)) return;  // error recorded; the force launch checks it again
         nbr_env = ctx->nbr_env.p;
     }
-    hipLaunchKernelGGL(k_pool<L>, dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx), nenv,
-                       off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->pool.p,
-                       ctx->npool, ctx->Aplus.p, estride, nbr_env);
+    const dim3 grid((unsigned)((nenv + epw - 1) / epw));
+    const int8_t* sp = ctx->nspecies > 0 ? species : nullptr;
+    if (epw == 2) {
+        set_func_lds((const void*)k_pool<L, 2>, lds);
+        hipLaunchKernelGGL((k_pool<L, 2>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
+                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+    } else {
+        set_func_lds((const void*)k_pool<L, 1>, lds);
+        hipLaunchKernelGGL((k_pool<L, 1>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
+                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+    }
 }
 
 // k_force: one wave per environment, lanes are neighbours.  The

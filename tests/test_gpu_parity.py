@@ -155,6 +155,23 @@ def test_basis_level(cuda):
     assert rel_err(dB, O.eval_dB(T, dAA)) < F_TOL
 
 
+@pytest.mark.parametrize("nenv,species", [(1, 0), (3, 0), (65, 0), (33, 2)])
+def test_two_environments_per_wave(cuda, nenv, species):
+    """Bases with at most 32 pooled values pool two environments per wave;
+    odd batch sizes leave the last half-wave without an environment."""
+    print(f"[gpu] two environments per wave nenv={nenv} species={species}", flush=True)
+    from acehip.pack import Pack
+    basis, T = basis_for(3, 6, 4) if species == 0 else basis_for(2, 4, 2, species=2)
+    assert len(Pack(T).pool) <= 32
+    c = synth.coefficients(len(basis), 9)
+    off, R, sp = random_batch(nenv, 50 + nenv, nmin=1, nmax=40, nspecies=species)
+    ev, E, dE = _run(basis, T, c, off, R, sp)
+    assert rel_err(E, O.energy(T, c, off, R, sp)) < E_TOL
+    assert rel_err(dE.T, O.grad_config(T, c, off, R, sp)) < F_TOL
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    assert rel_err(ev.basis_A(b).cpu().numpy(), O.eval_A(T, off, R, sp)) < E_TOL
+
+
 def test_empty_and_ragged(cuda):
     print("[gpu] empty environments", flush=True)
     basis, T = basis_for(3, 6, 4)
