@@ -1372,7 +1372,11 @@ struct WaveCfg {
 
 constexpr int FORCE_NRMAX = 16;  // radial functions held in registers by the force kernels
 constexpr int FF_QMAX = 256;  // (l, m >= 0, n) entries of the flat force kernel (npool <= 256)
-constexpr int FF_MAXE = 16;   // environments a 256-neighbour block stages in LDS
+// environments a 256-neighbour block stages in LDS: 32 when their G fits in
+// FF_LDS (short environments), else 16
+constexpr int FF_MAXE = 16;
+constexpr int FF_MAXE_WIDE = 32;
+constexpr size_t FF_LDS = 40 * 1024;
 
 struct ForceFlatTab {
     int8_t nrl[ACE_LMAX + 1];                                  // n bound per l
@@ -1995,7 +1999,7 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
 // every (l, m, n) in the kernel arguments.  Radial values stay in registers
 // and the ALP values are produced m-outer exactly as in k_force.
 
-template <int L, int NRM, bool STAGED>
+template <int L, int NRM, bool STAGED, int MAXE>
 __device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceFlatTab& ft, int64_t nnbr, int64_t j,
                                             const double* __restrict__ xyz, const double2* __restrict__ Ge,
                                             const double2* __restrict__ Gs, int64_t estride,
@@ -2113,7 +2117,7 @@ __device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceF
 #pragma unroll
             for (int n = 0; n < NRM; ++n) {
                 if (n < nb) {
-                    const double2 G = STAGED ? Gs[(q + n) * FF_MAXE] : Ge[(int64_t)sl[n] * estride];
+                    const double2 G = STAGED ? Gs[(q + n) * MAXE] : Ge[(int64_t)sl[n] * estride];
                     const double Rn = R[n], dRn = dR[n] * dtdr;
                     const double gy = G.x * Yr - G.y * Yi;
                     g0 += dRn * gy * rx + Rn * (G.x * dYr[0] - G.y * dYi[0]);
@@ -2129,32 +2133,32 @@ __device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceF
     dE[2 * nnbr + j] = g2;
 }
 
-template <int L, int NRM>
+template <int L, int NRM, int MAXE>
 __global__ __launch_bounds__(256, 4) void k_forcef(RadialParams rp, ForceFlatTab ft, int64_t nnbr,
                                                  const double* __restrict__ xyz,
                                                  const int32_t* __restrict__ nbr_env,
                                                  const int32_t* __restrict__ ffq,
                                                  const double2* __restrict__ Gbuf, int64_t estride,
                                                  double* __restrict__ dE) {
-    extern __shared__ double2 Gst[];  // [q][FF_MAXE]
+    extern __shared__ double2 Gst[];  // [q][MAXE]
     const int64_t jb = (int64_t)blockIdx.x * 256;
     const int64_t j = jb + threadIdx.x;
     const int64_t jl = min(jb + 255, nnbr - 1);
     // the block's environments (neighbours are grouped by environment)
     const int64_t e0 = nbr_env[jb];
     const int ne = (int)(nbr_env[jl] - e0 + 1);
-    if (ne <= FF_MAXE) {
+    if (ne <= MAXE) {
         for (int i = threadIdx.x; i < ft.nq * ne; i += 256) {
             const int qq = i / ne, el = i - qq * ne;
-            Gst[qq * FF_MAXE + el] = Gbuf[(int64_t)ffq[qq] * estride + e0 + el];
+            Gst[qq * MAXE + el] = Gbuf[(int64_t)ffq[qq] * estride + e0 + el];
         }
         __syncthreads();
         if (j >= nnbr) return;  // no barriers below
         const int el = (int)(nbr_env[j] - e0);
-        forcef_lane<L, NRM, true>(rp, ft, nnbr, j, xyz, nullptr, Gst + el, estride, dE);
+        forcef_lane<L, NRM, true, MAXE>(rp, ft, nnbr, j, xyz, nullptr, Gst + el, estride, dE);
     } else {
         if (j >= nnbr) return;
-        forcef_lane<L, NRM, false>(rp, ft, nnbr, j, xyz, Gbuf + nbr_env[j], nullptr, estride, dE);
+        forcef_lane<L, NRM, false, MAXE>(rp, ft, nnbr, j, xyz, Gbuf + nbr_env[j], nullptr, estride, dE);
     }
 }
 
@@ -2184,13 +2188,23 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
                                ctx->gfold.p, ctx->Gbuf.p, estride);
         // radial functions held in registers: 12 when they suffice (fewer
         // registers, more waves per SIMD), else 16
-        const size_t glds = (size_t)ctx->fflat.nq * FF_MAXE * sizeof(double2);
-        if (ctx->nr <= 12)
-            hipLaunchKernelGGL((k_forcef<L, 12>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), glds, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+        // blocks of short environments span more of them: stage 32 when that fits
+        const bool wide = (size_t)ctx->fflat.nq * FF_MAXE_WIDE * sizeof(double2) <= FF_LDS;
+        const size_t glds = (size_t)ctx->fflat.nq * (wide ? FF_MAXE_WIDE : FF_MAXE) * sizeof(double2);
+        const dim3 grid((unsigned)((nnbr + 255) / 256));
+        const int32_t* ne_ = ctx->nbr_env.p;
+        if (ctx->nr <= 12 && wide)
+            hipLaunchKernelGGL((k_forcef<L, 12, FF_MAXE_WIDE>), grid, dim3(256), glds, st, radial_params(ctx),
+                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+        else if (ctx->nr <= 12)
+            hipLaunchKernelGGL((k_forcef<L, 12, FF_MAXE>), grid, dim3(256), glds, st, radial_params(ctx),
+                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+        else if (wide)
+            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX, FF_MAXE_WIDE>), grid, dim3(256), glds, st,
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
         else
-            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX>), dim3((unsigned)((nnbr + 255) / 256)), dim3(256), glds, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ctx->nbr_env.p, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX, FF_MAXE>), grid, dim3(256), glds, st,
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
         return;
     }
     const int nsp = std::max(1, ctx->nspecies);

@@ -59,6 +59,24 @@ def test_config2_tile_parts(cuda, parts):
             assert rel_err(dE.T, O.grad_config(T, c, off, R)) < F_TOL
 
 
+@pytest.mark.parametrize("deg", [6, 12])
+def test_force_blocks_of_short_environments(cuda, deg):
+    """The flat force kernel stages the G of a 256-neighbour block's
+    environments in LDS (up to 32 of them for the small basis, 16 for the
+    order-4 one) and reads G from memory when a block spans more: one batch
+    holds blocks of both kinds."""
+    print(f"[gpu] force blocks of short environments deg={deg}", flush=True)
+    basis, T = basis_for(3, 6, 4) if deg == 6 else basis_for(4, 12, 6)
+    c = synth.coefficients(len(basis), 3)
+    o1, R1, _ = random_batch(150, 60, nmin=1, nmax=3)
+    o2, R2, _ = random_batch(30, 61, nmin=20, nmax=40)
+    off = np.concatenate([o1, o2[1:] + o1[-1]]).astype(np.int32)
+    R = np.concatenate([R1, R2])
+    _, E, dE = _run(basis, T, c, off, R, waves=16 if deg == 12 else 8)
+    assert rel_err(E, O.energy(T, c, off, R)) < E_TOL
+    assert rel_err(dE.T, O.grad_config(T, c, off, R)) < F_TOL
+
+
 def test_set_parts_rejects_bad_values(cuda):
     basis, T = basis_for(3, 6, 4)
     ev = HIPEvaluator(basis, synth.coefficients(len(basis), 1), device=0)
