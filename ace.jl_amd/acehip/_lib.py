@@ -46,7 +46,7 @@ class AcePack(C.Structure):
 
 
 # every symbol declared in include/acehip.h
-EXPORTS = ("ace_ctx_create", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_energy",
+EXPORTS = ("ace_ctx_create", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_set_tail_parts", "ace_energy",
            "ace_energy_forces", "ace_basis_A", "ace_basis_AA", "ace_basis_B",
            "ace_basis_dB", "ace_surr_features", "ace_surr_energy_forces",
            "ace_last_timing", "ace_set_validation", "ace_last_error",
@@ -77,6 +77,7 @@ def load(path: str | None = None):
     lib.ace_set_params.argtypes = [vp, _f64p, C.c_int64]
     lib.ace_set_tile.argtypes = [vp, C.c_int]
     lib.ace_set_parts.argtypes = [vp, C.c_int]
+    lib.ace_set_tail_parts.argtypes = [vp, C.c_int]
     ev = [vp, C.c_int64, vp, vp, vp, vp, vp]
     lib.ace_energy.argtypes = ev
     lib.ace_energy_forces.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]

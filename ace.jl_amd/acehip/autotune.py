@@ -4,7 +4,9 @@ Two shape parameters are free: the number of waves per workgroup (every wave
 interprets its share of the run programs for the same 64-environment tile;
 the pack carries a task assignment for each supported value) and the number
 of workgroups a tile's tasks are split over (1 to 4; Invariant bases),
-which shortens the last, partly empty round of workgroups.  The tuner times
+which shortens the last, partly empty round of workgroups.  Parts 0 is the
+library's per-call choice, which also gives the tiles of the last round one
+more part than the rest.  The tuner times
 every (waves, parts) pair on the caller's batch with events on the launch
 stream and keeps the fastest; results are cached per (basis size, batch size,
 mode).
@@ -24,7 +26,7 @@ def tune(ev, batch, forces: bool = True, reps: int = 3, candidates=None, parts=N
         ev.set_parts(P)
         return _CACHE[key]
     cands = list(candidates) if candidates else [int(w) for w in ev.pack.wave_cfg_W]
-    pcands = list(parts) if parts else ([1, 2, 3, 4] if ev.prop == 0 else [1])
+    pcands = list(parts) if parts else ([0, 1, 2, 3, 4] if ev.prop == 0 else [1])
     results = {}
     for W in cands:
         for P in pcands:

@@ -94,6 +94,11 @@ class HIPEvaluator:
         """Workgroups per product-DAG tile: 0 chooses per call (default), else 1 to 4."""
         _lib.check(self.lib.ace_set_parts(self.ctx, int(parts)))
 
+    def set_tail_parts(self, ntail: int = -1):
+        """Tiles at the end of the product-DAG grid that run one more part:
+        -1 chooses per call (automatic parts only), else that many."""
+        _lib.check(self.lib.ace_set_tail_parts(self.ctx, int(ntail)))
+
     def set_validation(self, on: bool):
         _lib.check(self.lib.ace_set_validation(self.ctx, int(bool(on))))
 

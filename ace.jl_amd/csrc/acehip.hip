@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -461,6 +462,11 @@ struct DagRecArgs {
     int npool;
     int W;
     int parts;                 // workgroups per tile, each running 1/parts of the tasks
+    // tiles [ntile_lo, ntile) run parts + 1 workgroups each, with these task
+    // lists: shorter workgroups that fill the grid's last round
+    int64_t ntile_lo;
+    const int32_t* wtask_ptr_hi;  // [(parts + 1) * W + 1]
+    const int32_t* wtask_list_hi;
 };
 
 // Streaming accesses: the A+ tile is read once and G written once per
@@ -543,8 +549,14 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // workgroup = (tile, part): the parts of a tile run disjoint task lists
-    const int64_t tile = blockIdx.x / a.parts;
-    const int part = (int)(blockIdx.x - tile * a.parts);
+    const int64_t blo = a.ntile_lo * a.parts;
+    const bool hi = (int64_t)blockIdx.x >= blo;
+    const int P = hi ? a.parts + 1 : a.parts;
+    const int64_t bt = hi ? (int64_t)blockIdx.x - blo : (int64_t)blockIdx.x;
+    const int64_t tile = (hi ? a.ntile_lo : 0) + bt / P;
+    const int part = (int)(bt % P);
+    const int32_t* const wptr = hi ? a.wtask_ptr_hi : a.wtask_ptr;
+    const int32_t* const wlist = hi ? a.wtask_list_hi : a.wtask_list;
     int64_t env = tile * 64 + lane;
     bool valid = env < nenv;
     if (env_list) {
@@ -560,9 +572,9 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
 
     const char* lds = reinterpret_cast<const char*>(As) + 16 * lane;
     double ep = 0.0;
-    const int tb = a.wtask_ptr[part * a.W + wave], te = a.wtask_ptr[part * a.W + wave + 1];
+    const int tb = wptr[part * a.W + wave], te = wptr[part * a.W + wave + 1];
     for (int ti = tb; ti < te; ++ti) {
-        const int t = a.wtask_list[ti];
+        const int t = wlist[ti];
         const int slot = a.tasks[5 * t + 2], rb = a.tasks[5 * t + 3], re = a.tasks[5 * t + 4];
         double2 acc = make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
         // P1, P2: prefix products of levels 1 and 2; Pc: the product of the
@@ -614,12 +626,14 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
 }
 
 // E = e0 + sum of the parts' energy shares (fixed order: deterministic).
-__global__ void k_esum(int64_t nenv, int parts, const double* __restrict__ Ep, int64_t estride, double e0,
-                       double* __restrict__ E_out) {
+// Environments from env_hi on belong to tiles with parts + 1 shares.
+__global__ void k_esum(int64_t nenv, int parts, int64_t env_hi, const double* __restrict__ Ep, int64_t estride,
+                       double e0, double* __restrict__ E_out) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nenv) return;
     double s = e0;
-    for (int p = 0; p < parts; ++p) s += Ep[(int64_t)p * estride + e];
+    const int np = e >= env_hi ? parts + 1 : parts;
+    for (int p = 0; p < np; ++p) s += Ep[(int64_t)p * estride + e];
     E_out[e] = s;
 }
 
@@ -1420,7 +1434,13 @@ struct ace_ctx {
     std::vector<WaveCfg*> wcfg;
     int cur_w = 0;
     int ncu = 256;  // compute units of the device (tail sizing of the product-DAG grid)
-    int parts = 0;  // workgroups per product-DAG tile: 0 = chosen per call, else 1, 2 or 4
+    int parts = 0;  // workgroups per product-DAG tile: 0 = chosen per call, else 1 to 4
+    // tiles at the end of the product-DAG grid that run one more part: -1 =
+    // chosen per call (automatic parts only), else that many
+    int tail = -1;
+    int64_t tail_key_ntile = -1;  // cache of the automatic choice
+    int tail_key_P = 0;
+    int64_t tail_key_n = 0;
     // last eagerly evaluated batch: its neighbour count serves calls under stream capture
     const int32_t* batch_off = nullptr;
     int64_t batch_nenv = -1, batch_nnbr = 0;
@@ -1684,6 +1704,48 @@ extern "C" int ace_set_params(ace_ctx* ctx, const double* c, int64_t nB) {
     if (nB != ctx->nB) return set_err("ace_set_params: len(c) != len(basis)");
     HIP_TRY(hipSetDevice(ctx->device));
     return upload_coeffs(ctx, c);
+}
+
+// Tail tiles for the automatic parts choice.  The grid runs in rounds of ncu
+// workgroups of equal length (1/P of a tile's work).  Giving the last n tiles
+// P + 1 parts fills the partial last round with shorter workgroups, each
+// 1/(P+1) of the work plus 8 % (re-staging and thinner task lists: measured
+// 0.200 ms per workgroup with 4 parts against 0.246 ms with 3 at 100k
+// environments).  The makespan of a greedy dispatch of such workgroups onto
+// ncu units is flat over a range of n; take the middle of that range.
+static int64_t choose_tail(int64_t ntile, int P, int ncu) {
+    const double d = 1.0 / P, dh = 1.08 / (P + 1);
+    std::vector<double> h((size_t)ncu);
+    auto makespan = [&](int64_t nhi) {
+        // the P-part workgroups: full rounds, then r units busy one more
+        const int64_t w = (ntile - nhi) * P, f = w / ncu, r = w % ncu;
+        for (int i = 0; i < ncu; ++i) h[(size_t)i] = (i < r ? f + 1 : f) * d;
+        std::make_heap(h.begin(), h.end(), std::greater<double>());
+        for (int64_t k = 0; k < nhi * (P + 1); ++k) {
+            std::pop_heap(h.begin(), h.end(), std::greater<double>());
+            h.back() += dh;
+            std::push_heap(h.begin(), h.end(), std::greater<double>());
+        }
+        return *std::max_element(h.begin(), h.end());
+    };
+    const int64_t nmax = std::min<int64_t>(ntile, ncu);
+    std::vector<double> ms((size_t)nmax + 1);
+    double best = 1e300;
+    for (int64_t n = 0; n <= nmax; ++n) {
+        ms[(size_t)n] = makespan(n);
+        best = std::min(best, ms[(size_t)n]);
+    }
+    std::vector<int64_t> ties;
+    for (int64_t n = 0; n <= nmax; ++n)
+        if (ms[(size_t)n] <= best * (1.0 + 1e-9)) ties.push_back(n);
+    return ties[ties.size() / 2];
+}
+
+extern "C" int ace_set_tail_parts(ace_ctx* ctx, int ntail) {
+    if (!ctx) return set_err("ace_set_tail_parts: null context");
+    if (ntail < -1) return set_err("ace_set_tail_parts: ntail must be -1 (automatic) or >= 0");
+    ctx->tail = ntail;
+    return 0;
 }
 
 extern "C" int ace_set_parts(ace_ctx* ctx, int parts) {
@@ -2309,6 +2371,20 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
             }
         }
         ctx->last_parts = P;
+        // tail tiles with P + 1 parts (P >= 2, so every tile sums shares in k_esum)
+        int64_t nhi = 0;
+        if (ctx->tail >= 0) {
+            nhi = std::min<int64_t>(ctx->tail, ntile);
+            if (nhi > 0 && (P < 2 || P >= MAX_PARTS))
+                return set_err("ace_set_tail_parts: tail tiles need 2 or 3 parts per tile (ace_set_parts)");
+        } else if (ctx->parts == 0 && P >= 2 && P + 1 <= MAX_PARTS && ntask_mode >= 2 * (P + 1) * wc->W) {
+            if (ctx->tail_key_ntile != ntile || ctx->tail_key_P != P) {
+                ctx->tail_key_ntile = ntile;
+                ctx->tail_key_P = P;
+                ctx->tail_key_n = choose_tail(ntile, P, ctx->ncu);
+            }
+            nhi = ctx->tail_key_n;
+        }
         DagRecArgs ra;
         ra.recs = ctx->recs.p;
         ra.tasks = ctx->d_tasks.p;
@@ -2318,14 +2394,19 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
         ra.npool = ctx->npool;
         ra.W = wc->W;
         ra.parts = P;
-        if (P > 1 && ctx->Epart.ensure((size_t)P * estride)) return -1;
+        ra.ntile_lo = ntile - nhi;
+        ra.wtask_ptr_hi = nhi > 0 ? (forces ? wc->pp_all[P + 1].p : wc->pp_e[P + 1].p) : nullptr;
+        ra.wtask_list_hi = nhi > 0 ? (forces ? wc->pl_all[P + 1].p : wc->pl_e[P + 1].p) : nullptr;
+        const int nshare = nhi > 0 ? P + 1 : P;
+        if (P > 1 && ctx->Epart.ensure((size_t)nshare * estride)) return -1;
+        const int64_t nblk = (ntile - nhi) * P + nhi * (P + 1);
         set_func_lds((const void*)k_dagr, lds);
-        hipLaunchKernelGGL(k_dagr, dim3((unsigned)(ntile * P)), dim3(64 * wc->W), lds, st, ra, nenv,
+        hipLaunchKernelGGL(k_dagr, dim3((unsigned)nblk), dim3(64 * wc->W), lds, st, ra, nenv,
                            ctx->Aplus.p, estride, P == 1 ? E_out : ctx->Epart.p, ctx->Gbuf.p, ctx->e0r,
                            (const int32_t*)nullptr, (const int32_t*)nullptr);
         if (P > 1)
-            hipLaunchKernelGGL(k_esum, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, P, ctx->Epart.p,
-                               estride, ctx->e0r, E_out);
+            hipLaunchKernelGGL(k_esum, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, P,
+                               (ntile - nhi) * 64, ctx->Epart.p, estride, ctx->e0r, E_out);
     } else if ((size_t)ctx->npool * 64 * sizeof(double2) <= 160 * 1024) {
         set_func_lds((const void*)k_dag<1, 64>, lds);
         hipLaunchKernelGGL((k_dag<1, 64>), dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, a, nenv,
@@ -2667,6 +2748,9 @@ extern "C" int ace_surr_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t*
     ra.npool = ctx->npool;
     ra.W = wc->W;
     ra.parts = 1;
+    ra.ntile_lo = ntile;
+    ra.wtask_ptr_hi = nullptr;
+    ra.wtask_list_hi = nullptr;
     size_t lds = std::max((size_t)ctx->npool * 64 * sizeof(double2), (size_t)wc->W * 64 * sizeof(double));
     if (lds > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     set_func_lds((const void*)k_dagr, lds);

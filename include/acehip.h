@@ -149,6 +149,13 @@ int ace_set_tile(ace_ctx* ctx, int waves_per_tile);
  * 0 (default) chooses 1 to 4 per call from the tile and CU counts. */
 int ace_set_parts(ace_ctx* ctx, int parts);
 
+/* Tiles at the end of the product-DAG grid (Invariant) that run one more
+ * part than the rest, so that the last round holds shorter workgroups.
+ * -1 (default) chooses the count per call when the parts are automatic;
+ * n >= 0 uses n tiles and needs 2 or 3 parts per tile.  A tuning knob of
+ * this implementation, like ace_set_parts; no counterpart in the reference. */
+int ace_set_tail_parts(ace_ctx* ctx, int ntail);
+
 /* Site energies, ProductEvaluator evaluate (evaluator.jl:121-147), batched.
  * E_out[nenv] (Invariant) or E_out[nenv][3] (EuclideanVector).  species
  * [nnbr] int8 category index (NULL without a categorical 1p basis). */

@@ -77,6 +77,52 @@ def test_force_blocks_of_short_environments(cuda, deg):
     assert rel_err(dE.T, O.grad_config(T, c, off, R)) < F_TOL
 
 
+_TAIL_REF = {}
+
+
+@pytest.mark.parametrize("parts,tail", [(2, 1), (3, 2), (2, 3), (3, 9)])
+def test_tail_tiles_with_one_more_part(cuda, parts, tail):
+    """The last `tail` tiles of the product-DAG grid run parts + 1 workgroups
+    with the (parts + 1)-way task split; their energies sum parts + 1 shares.
+    200 environments are 4 tiles (the last one partial); tail 9 covers all."""
+    print(f"[gpu] tail tiles parts={parts} tail={tail}", flush=True)
+    basis, T = basis_for(4, 12, 6)
+    c = synth.coefficients(len(basis), 2)
+    off, R, sp = synth.workload_envs(2, nenv=200)
+    if not _TAIL_REF:
+        _TAIL_REF["E"] = O.energy(T, c, off, R)
+        _TAIL_REF["g"] = O.grad_config(T, c, off, R)
+    ev = HIPEvaluator(basis, c, device=0, waves=16)
+    ev.set_parts(parts)
+    ev.set_tail_parts(tail)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    E2 = ev.energy(b)
+    torch.cuda.synchronize()
+    assert rel_err(E.cpu().numpy(), _TAIL_REF["E"]) < E_TOL
+    assert rel_err(E2.cpu().numpy(), _TAIL_REF["E"]) < E_TOL
+    assert rel_err(dE.cpu().numpy().T, _TAIL_REF["g"]) < F_TOL
+
+
+def test_tail_parts_rejects_bad_settings(cuda):
+    from acehip._lib import AceError
+    basis, T = basis_for(3, 6, 4)
+    ev = HIPEvaluator(basis, synth.coefficients(len(basis), 1), device=0)
+    with pytest.raises(AceError):
+        ev.set_tail_parts(-2)
+    off, R, _ = random_batch(70, 3)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    for parts in (1, 4):
+        ev.set_parts(parts)
+        ev.set_tail_parts(1)
+        with pytest.raises(AceError, match="tail"):
+            ev.energy(b)
+    ev.set_tail_parts(0)
+    E = ev.energy(b)
+    torch.cuda.synchronize()
+    assert rel_err(E.cpu().numpy(), O.energy(T, synth.coefficients(len(basis), 1), off, R)) < E_TOL
+
+
 def test_set_parts_rejects_bad_values(cuda):
     basis, T = basis_for(3, 6, 4)
     ev = HIPEvaluator(basis, synth.coefficients(len(basis), 1), device=0)
