@@ -24,12 +24,12 @@ METRIC = "env/s energy+forces: plain FP64 vs surrogate at ≤1 meV/atom MAE"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_dagr launch on this workload (100k environments, 16-wave
-# tiles split over 2 workgroups), from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE runs of this script, scaled by the factors a 256 MiB device copy
-# gives the same counters (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE): 550.4 MB
-# read + 269.1 MB written.  Only valid for the default --nenv; re-measured
-# whenever the kernel changes.
-DAG_TRAFFIC_BYTES = {100_000: 966.3e6}
+# tiles split over 3 workgroups, 4 for the tiles of the last round), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the
+# factors a 256 MiB device copy gives the same counters (2.0 for FETCH_SIZE,
+# 1.0 for WRITE_SIZE): 714.9 MB read + 269.7 MB written.  Only valid for the
+# default --nenv; re-measured whenever the kernel changes.
+DAG_TRAFFIC_BYTES = {100_000: 984.5e6}
 TRAFFIC_SOURCE = "profiles/plain_ef_config2_kernels.md"
 
 
