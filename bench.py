@@ -303,8 +303,11 @@ def main():
                                "order 4, degree 12, Lmax 6, Invariant, FP64",
                    "nenv": b.nenv, "nnbr": b.nnbr, "order": 4, "maxdeg": 12, "L": 6,
                    "len_B": len(basis), "len_AA": len(pk.orders), "npool": len(pk.pool),
-                   "tile_waves": ev.waves, "tile_parts": None if tuned is None else tuned[0][1],
-                   "autotune_ms": None if tuned is None else {f"{k[0]}x{k[1]}": round(v, 3)
+                   "tile_waves": ev.waves,
+                   # 0: chosen per call (parts from the tile and CU counts, and
+                   # one more part for the tiles of the grid's last round)
+                   "tile_parts": ("automatic" if tuned is None or tuned[0][1] == 0 else tuned[0][1]),
+                   "autotune_ms": None if tuned is None else {f"{k[0]}x{k[1] or 'auto'}": round(v, 3)
                                                               for k, v in tuned[1].items()}},
         "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (no MFMA); MI355X FP64 peak",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
