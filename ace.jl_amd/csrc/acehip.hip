@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <map>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1438,9 +1439,7 @@ struct ace_ctx {
     // tiles at the end of the product-DAG grid that run one more part: -1 =
     // chosen per call (automatic parts only), else that many
     int tail = -1;
-    int64_t tail_key_ntile = -1;  // cache of the automatic choice
-    int tail_key_P = 0;
-    int64_t tail_key_n = 0;
+    std::map<std::pair<int64_t, int>, int64_t> tail_cache;  // automatic choice per (tiles, parts)
     // last eagerly evaluated batch: its neighbour count serves calls under stream capture
     const int32_t* batch_off = nullptr;
     int64_t batch_nenv = -1, batch_nnbr = 0;
@@ -2378,12 +2377,13 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
             if (nhi > 0 && (P < 2 || P >= MAX_PARTS))
                 return set_err("ace_set_tail_parts: tail tiles need 2 or 3 parts per tile (ace_set_parts)");
         } else if (ctx->parts == 0 && P >= 2 && P + 1 <= MAX_PARTS && ntask_mode >= 2 * (P + 1) * wc->W) {
-            if (ctx->tail_key_ntile != ntile || ctx->tail_key_P != P) {
-                ctx->tail_key_ntile = ntile;
-                ctx->tail_key_P = P;
-                ctx->tail_key_n = choose_tail(ntile, P, ctx->ncu);
+            const auto key = std::make_pair(ntile, P);
+            auto it = ctx->tail_cache.find(key);
+            if (it == ctx->tail_cache.end()) {
+                if (ctx->tail_cache.size() >= 64) ctx->tail_cache.clear();
+                it = ctx->tail_cache.emplace(key, choose_tail(ntile, P, ctx->ncu)).first;
             }
-            nhi = ctx->tail_key_n;
+            nhi = it->second;
         }
         DagRecArgs ra;
         ra.recs = ctx->recs.p;

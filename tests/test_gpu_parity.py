@@ -104,6 +104,36 @@ def test_tail_tiles_with_one_more_part(cuda, parts, tail):
     assert rel_err(dE.cpu().numpy().T, _TAIL_REF["g"]) < F_TOL
 
 
+def test_tail_tiles_full_size(cuda):
+    """At 100k environments (1,563 tiles) the automatic choice gives the tiles
+    of the grid's last round one more part.  Energies and gradients equal the
+    run without tail tiles, and on the environments of the last tile the
+    oracle."""
+    print("[gpu] tail tiles at 100k environments", flush=True)
+    basis, T = basis_for(4, 12, 6)
+    c = synth.coefficients(len(basis), 2)
+    off, R, _ = synth.workload_envs(2, nenv=100000)
+    ev = HIPEvaluator(basis, c, device=0, waves=16)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    runs = {}
+    for parts, tail in ((0, 0), (0, -1), (3, 128)):
+        ev.set_parts(parts)
+        ev.set_tail_parts(tail)
+        E, dE = ev.energy_forces(b)
+        torch.cuda.synchronize()
+        runs[(parts, tail)] = (E.cpu().numpy(), dE.cpu().numpy())
+    E0, g0 = runs[(0, 0)]
+    for key, (E, g) in runs.items():
+        assert rel_err(E, E0) < 1e-14, key
+        assert rel_err(g, g0) < 1e-14, key
+    e0 = 100000 - 24
+    o = (off[e0:] - off[e0]).astype(np.int32)
+    Rs = R[off[e0]:]
+    Et, gt = runs[(3, 128)]
+    assert rel_err(Et[e0:], O.energy(T, c, o, Rs)) < E_TOL
+    assert rel_err(gt[:, off[e0]:].T, O.grad_config(T, c, o, Rs)) < F_TOL
+
+
 def test_tail_parts_rejects_bad_settings(cuda):
     from acehip._lib import AceError
     basis, T = basis_for(3, 6, 4)
