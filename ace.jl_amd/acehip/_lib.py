@@ -13,10 +13,26 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libacehip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
 _f64p = C.POINTER(C.c_double)
+
+COMP_CATEGORICAL, COMP_RN, COMP_YLM = 0, 1, 2  # ACE_COMP_* of include/acehip.h
+
+
+class AceJlBasis(C.Structure):
+    """``ace_jl_basis``: a basis in ACE.jl's own memory layout."""
+    _fields_ = [
+        ("abi_version", C.c_int32), ("prop", C.c_int32),
+        ("nr", C.c_int64), ("A", _f64p), ("B", _f64p), ("C", _f64p),
+        ("pl", C.c_int64), ("pr", C.c_int64), ("tl", C.c_double), ("tr", C.c_double),
+        ("tp", C.c_int64), ("r0", C.c_double), ("maxL", C.c_int64), ("ncategories", C.c_int64),
+        ("ncomp", C.c_int64), ("comp_kind", _i32p), ("nA", C.c_int64), ("indices", _i64p),
+        ("nAA", C.c_int64), ("maxorder", C.c_int64), ("orders", _i64p), ("iAA2iA", _i64p),
+        ("nB", C.c_int64), ("colptr", _i64p), ("rowval", _i64p), ("nzval", _f64p),
+    ]
 
 
 class AcePack(C.Structure):
@@ -46,7 +62,7 @@ class AcePack(C.Structure):
 
 
 # every symbol declared in include/acehip.h
-EXPORTS = ("ace_ctx_create", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_set_tail_parts", "ace_energy",
+EXPORTS = ("ace_ctx_create", "ace_ctx_create_jl", "ace_compile_check", "ace_last_dag_shape", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_set_tail_parts", "ace_energy",
            "ace_energy_forces", "ace_basis_A", "ace_basis_AA", "ace_basis_B",
            "ace_basis_dB", "ace_surr_features", "ace_surr_energy_forces",
            "ace_last_timing", "ace_set_validation", "ace_last_error",
@@ -74,6 +90,9 @@ def load(path: str | None = None):
     lib = C.CDLL(p)
     vp = C.c_void_p
     lib.ace_ctx_create.argtypes = [C.POINTER(AcePack), C.c_int, _f64p, C.c_int64, C.POINTER(vp)]
+    lib.ace_ctx_create_jl.argtypes = [C.POINTER(AceJlBasis), C.c_int, _f64p, C.c_int64, C.POINTER(vp)]
+    lib.ace_compile_check.argtypes = [C.POINTER(AceJlBasis), _i64p]
+    lib.ace_last_dag_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]
     lib.ace_set_params.argtypes = [vp, _f64p, C.c_int64]
     lib.ace_set_tile.argtypes = [vp, C.c_int]
     lib.ace_set_parts.argtypes = [vp, C.c_int]
@@ -190,4 +209,75 @@ def make_pack(pk):
     s.wave_task_list = _ptr(arr(pk.wave_task_list, np.int32), C.c_int32)
     s.wave_etask_ptr = _ptr(arr(pk.wave_etask_ptr, np.int32), C.c_int32)
     s.wave_etask_list = _ptr(arr(pk.wave_etask_list, np.int32), C.c_int32)
+    return s, keep
+
+
+def jl_arrays(T: dict) -> dict:
+    """The basis tables ``T`` (acehip.tables.basis_tables) in the layout ACE.jl
+    keeps them in (what a Julia caller passes to ``ace_ctx_create_jl``):
+    ``indices[nA][ncomp]`` int64 of Product1pBasis (category index, n,
+    index_y(l, m) = m + l + l^2 + 1; 1-based), ``iAA2iA`` column-major and
+    1-based with 0 padding, the A2B map as SparseMatrixCSC (colptr, rowval
+    1-based; nzval real, or 3 complex values re/im interleaved)."""
+    aspec = np.asarray(T["aspec"], dtype=np.int64).reshape(-1, 4)
+    cat = T["categories"] is not None
+    kinds = ([COMP_CATEGORICAL] if cat else []) + [COMP_RN, COMP_YLM]
+    cols = ([aspec[:, 0] + 1] if cat else []) + [aspec[:, 1], aspec[:, 3] + aspec[:, 2] + aspec[:, 2] ** 2 + 1]
+    indices = np.stack(cols, axis=1).astype(np.int64)
+    orders = np.asarray(T["orders"], dtype=np.int64)
+    iAA2iA = np.asarray(T["iAA2iA"], dtype=np.int64) + 1          # -1 padding -> 0
+    a2b = T["a2b"]
+    nB, nAA = (int(x) for x in a2b["shape"])
+    rows = np.asarray(a2b["rows"], dtype=np.int64)
+    ccol = np.asarray(a2b["cols"], dtype=np.int64)
+    order = np.lexsort((rows, ccol))                              # column-major, rows ascending
+    rows, ccol = rows[order], ccol[order]
+    vals = np.asarray(a2b["vals"])[order]
+    colptr = np.zeros(nAA + 1, dtype=np.int64)
+    np.add.at(colptr, ccol + 1, 1)
+    colptr = np.cumsum(colptr) + 1
+    if T["prop"] == "Invariant":
+        nz = np.asarray(vals, dtype=np.float64)
+    else:
+        v = np.asarray(vals, dtype=np.complex128).reshape(-1, 3)
+        nz = np.stack([v.real, v.imag], axis=-1).reshape(-1)
+    rn = T["rn"]
+    return {"prop": 0 if T["prop"] == "Invariant" else 1, "A": np.asarray(rn["A"], dtype=np.float64),
+            "B": np.asarray(rn["B"], dtype=np.float64), "C": np.asarray(rn["C"], dtype=np.float64),
+            "pl": int(rn["pl"]), "pr": int(rn["pr"]), "tl": float(rn["tl"]), "tr": float(rn["tr"]),
+            "tp": int(rn["p"]), "r0": float(rn["r0"]), "maxL": int(T["L"]),
+            "ncategories": len(T["categories"]) if cat else 0, "comp_kind": np.asarray(kinds, dtype=np.int32),
+            "indices": indices, "orders": orders, "iAA2iA": np.asfortranarray(iAA2iA).T.copy(), "nB": nB,
+            "colptr": colptr, "rowval": rows + 1, "nzval": nz}
+
+
+def make_jl_basis(T: dict):
+    """Fill an AceJlBasis from the basis tables; returns (struct, keepalive)."""
+    J = jl_arrays(T)
+    keep = []
+
+    def arr(x, dt, ct):
+        a = np.ascontiguousarray(x, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(ct))
+
+    s = AceJlBasis()
+    s.abi_version = ABI_VERSION
+    s.prop = J["prop"]
+    s.nr = len(J["A"])
+    s.A, s.B, s.C = (arr(J[k], np.float64, C.c_double) for k in ("A", "B", "C"))
+    s.pl, s.pr, s.tl, s.tr, s.tp, s.r0 = J["pl"], J["pr"], J["tl"], J["tr"], J["tp"], J["r0"]
+    s.maxL, s.ncategories = J["maxL"], J["ncategories"]
+    s.ncomp = len(J["comp_kind"])
+    s.comp_kind = arr(J["comp_kind"], np.int32, C.c_int32)
+    s.nA = J["indices"].shape[0]
+    s.indices = arr(J["indices"], np.int64, C.c_int64)
+    s.nAA = len(J["orders"])
+    s.maxorder = J["iAA2iA"].shape[0]
+    s.orders = arr(J["orders"], np.int64, C.c_int64)
+    s.iAA2iA = arr(J["iAA2iA"], np.int64, C.c_int64)        # [maxorder][nAA]: Julia's column-major matrix
+    s.nB = J["nB"]
+    s.colptr = arr(J["colptr"], np.int64, C.c_int64)
+    s.rowval = arr(J["rowval"], np.int64, C.c_int64)
+    s.nzval = arr(J["nzval"], np.float64, C.c_double)
     return s, keep

@@ -110,13 +110,14 @@ def site_energies(ev: HIPEvaluator, c: torch.Tensor, offsets, xyz: torch.Tensor,
     """
     if not torch.is_tensor(xyz) or xyz.dtype != torch.float64 or xyz.dim() != 2 or xyz.shape[0] != 3:
         raise ValueError("xyz must be a float64 tensor of shape (3, nnbr)")
-    if not torch.is_tensor(c) or c.dtype != torch.float64 or c.shape != (ev.pack.nB,):
-        raise ValueError(f"c must be a float64 tensor of shape ({ev.pack.nB},)")
-    ch = c.detach().cpu().numpy()
+    if not torch.is_tensor(c) or c.dtype != torch.float64 or c.shape != (ev.nB,):
+        raise ValueError(f"c must be a float64 tensor of shape ({ev.nB},)")
+    # upload c when it differs from the coefficients on the device: compared on
+    # the device, so the usual case (unchanged c) needs no host round trip
     last = getattr(ev, "_autograd_c", None)
-    if last is None or not np.array_equal(last, ch):
-        ev.set_params(ch)
-        ev._autograd_c = ch.copy()
+    if last is None or last.shape != c.shape or last.device != c.device or not bool(torch.equal(last, c.detach())):
+        ev.set_params(c.detach().cpu().numpy())
+        ev._autograd_c = c.detach().clone()
     b = EnvBatch(offsets, xyz.detach(), species, device=xyz.device)
     return _SiteEnergy.apply(c, xyz, ev, b)
 

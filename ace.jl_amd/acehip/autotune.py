@@ -19,13 +19,13 @@ _CACHE: dict = {}
 
 
 def tune(ev, batch, forces: bool = True, reps: int = 3, candidates=None, parts=None):
-    key = (len(ev.pack.orders), len(ev.pack.pool), batch.nenv, forces)
+    key = (ev.nAA, ev.nA, batch.nenv, forces)
     if key in _CACHE:
         W, P = _CACHE[key][0]
         ev.set_tile(W)
         ev.set_parts(P)
         return _CACHE[key]
-    cands = list(candidates) if candidates else [int(w) for w in ev.pack.wave_cfg_W]
+    cands = list(candidates) if candidates else [4, 8, 16]
     pcands = list(parts) if parts else ([0, 1, 2, 3, 4] if ev.prop == 0 else [1])
     results = {}
     for W in cands:

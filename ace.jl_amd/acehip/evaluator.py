@@ -63,22 +63,53 @@ def _vp(t):
 
 
 class HIPEvaluator:
-    """Context of libacehip.so for one SymmetricBasis and coefficients."""
+    """Context of libacehip.so for one SymmetricBasis and coefficients.
 
-    def __init__(self, basis, c, device: int = 0, waves: int = 8):
+    ``compiler="native"`` (default) hands the library the basis in ACE.jl's
+    own layout (``ace_ctx_create_jl``: Product1pBasis indices, PIBasisSpec,
+    A2Bmap as CSC), and the library compiles the device programs itself, as
+    it does for a Julia caller.  ``compiler="python"`` builds the same tables
+    with the Python packer (:class:`acehip.pack.Pack`) and passes them to
+    ``ace_ctx_create``; both give identical tables."""
+
+    def __init__(self, basis, c, device: int = 0, waves: int = 8, compiler: str = "native"):
         self.lib = _lib.load()
         self.basis = basis
         self.T = basis_tables(basis)
-        self.pack = Pack(self.T)
-        self.prop = self.pack.prop
+        self._pack = None
+        self.prop = 0 if self.T["prop"] == "Invariant" else 1
+        self.nB = int(self.T["a2b"]["shape"][0])
+        self.nA = len(self.T["aspec"])
+        self.nAA = len(self.T["orders"])
         self.device = device
-        struct, keep = _lib.make_pack(self.pack)
         c = np.ascontiguousarray(c, dtype=np.float64)
         ctx = C.c_void_p()
-        _lib.check(self.lib.ace_ctx_create(C.byref(struct), device, c.ctypes.data_as(_lib._f64p),
-                                           len(c), C.byref(ctx)))
+        if compiler == "native":
+            struct, keep = _lib.make_jl_basis(self.T)
+            _lib.check(self.lib.ace_ctx_create_jl(C.byref(struct), device, c.ctypes.data_as(_lib._f64p),
+                                                  len(c), C.byref(ctx)))
+        elif compiler == "python":
+            struct, keep = _lib.make_pack(self.pack)
+            _lib.check(self.lib.ace_ctx_create(C.byref(struct), device, c.ctypes.data_as(_lib._f64p),
+                                               len(c), C.byref(ctx)))
+        else:
+            raise ValueError("compiler must be 'native' or 'python'")
         self.ctx = ctx
         self.set_tile(waves)
+
+    @property
+    def pack(self):
+        """The Python packer's tables of this basis (built on first use)."""
+        if self._pack is None:
+            self._pack = Pack(self.T)
+        return self._pack
+
+    def last_dag_shape(self):
+        """Product-DAG launch of the last call: (parts per tile, tail tiles,
+        kernel: 0 records, else the run interpreter's environments per tile)."""
+        p, t, k = C.c_int(), C.c_int64(), C.c_int()
+        _lib.check(self.lib.ace_last_dag_shape(self.ctx, C.byref(p), C.byref(t), C.byref(k)))
+        return p.value, t.value, k.value
 
     def __del__(self):
         ctx = getattr(self, "ctx", None)
@@ -105,6 +136,7 @@ class HIPEvaluator:
     def set_params(self, c):
         c = np.ascontiguousarray(c, dtype=np.float64)
         _lib.check(self.lib.ace_set_params(self.ctx, c.ctypes.data_as(_lib._f64p), len(c)))
+        self._autograd_c = None  # the coefficient cache of acehip.autograd no longer matches
 
     def _stream(self, stream):
         if stream is None:
@@ -139,13 +171,13 @@ class HIPEvaluator:
 
     # -- basis level
     def basis_A(self, b: EnvBatch, stream=None):
-        out = torch.empty((b.nenv, self.pack.aspec_fid.size, 2), dtype=torch.float64, device=b.xyz.device)
+        out = torch.empty((b.nenv, self.nA, 2), dtype=torch.float64, device=b.xyz.device)
         _lib.check(self.lib.ace_basis_A(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
                                         _vp(out), self._stream(stream)))
         return torch.view_as_complex(out)
 
     def basis_AA(self, b: EnvBatch, stream=None):
-        nAA = len(self.pack.orders)
+        nAA = self.nAA
         if self.prop == 0:
             out = torch.empty((b.nenv, nAA), dtype=torch.float64, device=b.xyz.device)
         else:
@@ -155,14 +187,14 @@ class HIPEvaluator:
         return out if self.prop == 0 else torch.view_as_complex(out)
 
     def basis_B(self, b: EnvBatch, stream=None):
-        shape = (b.nenv, self.pack.nB) if self.prop == 0 else (b.nenv, self.pack.nB, 3)
+        shape = (b.nenv, self.nB) if self.prop == 0 else (b.nenv, self.nB, 3)
         out = torch.empty(shape, dtype=torch.float64, device=b.xyz.device)
         _lib.check(self.lib.ace_basis_B(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
                                         _vp(out), self._stream(stream)))
         return out
 
     def basis_dB(self, b: EnvBatch, stream=None):
-        shape = (b.nnbr, self.pack.nB, 3) if self.prop == 0 else (b.nnbr, self.pack.nB, 3, 3)
+        shape = (b.nnbr, self.nB, 3) if self.prop == 0 else (b.nnbr, self.nB, 3, 3)
         out = torch.zeros(shape, dtype=torch.float64, device=b.xyz.device)
         _lib.check(self.lib.ace_basis_dB(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
                                          _vp(out), self._stream(stream)))
@@ -174,7 +206,7 @@ class HIPEvaluator:
         w = w.to(device=b.xyz.device, dtype=torch.float64).contiguous()
         if tuple(w.shape) != (3, b.nnbr):
             raise ValueError("w must have shape (3, nnbr)")
-        out = torch.empty((b.nenv, self.pack.nB), dtype=torch.float64, device=b.xyz.device)
+        out = torch.empty((b.nenv, self.nB), dtype=torch.float64, device=b.xyz.device)
         _lib.check(self.lib.ace_adjoint_EVAL_D(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
                                                _vp(w), _vp(out), self._stream(stream)))
         return out

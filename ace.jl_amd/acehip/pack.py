@@ -16,6 +16,7 @@ WAVE_CONFIGS = (4, 8, 16)     # waves per workgroup of the product-DAG kernel
 ENERGY_PARTS = 16             # energy polynomial split into this many tasks
 REC_LEAVES = 4                # leaves per 64-byte program record
 MAX_TASK_COST = 1200.0        # gradient polynomials above this are split (tasks fit a quarter-tile wave)
+MAX_FEATURES = 160            # order <= 2 invariants the surrogate kernels take
 
 
 class Pack:
@@ -29,8 +30,8 @@ class Pack:
         self.nspecies = 0 if T["categories"] is None else len(T["categories"])
         orders = np.asarray(T["orders"], dtype=np.int64)
         iAA2iA = np.asarray(T["iAA2iA"], dtype=np.int64)
-        if iAA2iA.shape[1] > 4:
-            raise ValueError("correlation order above 4 is not supported")
+        if iAA2iA.shape[1] > P.MAX_ORDER:
+            raise ValueError("correlation order above 8 is not supported")
         is_m0 = self.pool[:, 3] == 0
         poly, _ = P.energy_polynomial(orders, iAA2iA, kk, cj, sgn)
         poly = P.merge_conjugates(poly, is_m0)
@@ -59,10 +60,10 @@ class Pack:
         self.gslot_list = np.asarray([s for ss in gslots for s in ss], dtype=np.int32)
         self.prog = prog
         # flatten programs
-        self.runs = np.asarray(prog.runs, dtype=np.int32).reshape(-1, 7)
+        self.runs = np.asarray(prog.runs, dtype=np.int32).reshape(-1, P.RUN_W)
         self.leaf_fid = np.asarray(prog.leaf_fid, dtype=np.int32)
         tasks3 = np.asarray(prog.tasks, dtype=np.int32).reshape(-1, 3)
-        self._build_records(tasks3)
+        self._build_records(tasks3, records=iAA2iA.shape[1] <= 4)
         lt_ptr, rows, scales, conjs = [0], [], [], []
         for terms in prog.leaf_terms:
             for (r, s, c) in terms:
@@ -154,6 +155,9 @@ class Pack:
         maxo = np.zeros(self.nB, dtype=np.int64)
         np.maximum.at(maxo, rows, o)
         feat_rows = np.nonzero((maxo >= 1) & (maxo <= 2))[0]
+        if len(feat_rows) > MAX_FEATURES:
+            self._empty_features()  # more invariants than the surrogate kernels take
+            return
         self.feat_rows = feat_rows.astype(np.int32)
         self.const_rows = np.nonzero(maxo == 0)[0].astype(np.int32)
         fptr, ff, fc = [0], [], []
@@ -201,21 +205,23 @@ class Pack:
     def _is_m0(self, f):
         return self.pool[f & 0x7fff, 3] == 0
 
-    def _build_records(self, tasks3):
+    def _build_records(self, tasks3, records=True):
         """64-byte records: 8 header dwords + REC_LEAVES coefficients.
 
         dw0 = depth | jstart << 2 | nleaf << 5; dw1 = f1 | f2 << 16;
         dw2 = f3; dw3.. = leaf factor ids, two per dword.  A run with more
         than REC_LEAVES leaves continues in further records with the same
         prefix (jstart = depth + 1).  ``rec_leaf[r, i]`` is the leaf whose
-        coefficient fills slot i (-1: empty)."""
+        coefficient fills slot i (-1: empty).  Records hold three prefix
+        factors: bases above correlation order 4 get none (the run
+        interpreter evaluates them)."""
         K = REC_LEAVES
         hdr, rleaf = [], []
         rec_of_task = []
         for (r0, r1, slot) in tasks3:
             b = len(hdr)
-            for r in range(r0, r1):
-                d, j, f1, f2, f3, nl, lo = (int(x) for x in self.runs[r])
+            for r in range(r0, r1 if records else r0):
+                d, j, nl, lo, f1, f2, f3 = (int(x) for x in self.runs[r][:7])
                 for q0 in range(0, max(nl, 1), K):
                     n = min(K, nl - q0)
                     jj = j if q0 == 0 else d + 1
@@ -243,7 +249,7 @@ class Pack:
         for t in tasks:
             r0, r1 = self.tasks[t][0], self.tasks[t][1]
             rr = self.runs[r0:r1]
-            leaves += int(rr[:, 5].sum())
+            leaves += int(rr[:, 2].sum())
             runs += len(rr)
             newf += int(np.sum(np.maximum(rr[:, 0] - rr[:, 1] + 1, 0)))
         recs = sum(int(self.tasks[t][4] - self.tasks[t][3]) for t in tasks)

@@ -32,6 +32,8 @@ from collections import defaultdict
 import numpy as np
 
 DEV_CONJ = 0x8000
+RUN_W = 11        # int32 words per run (ACE_RUN_W of the C ABI)
+MAX_ORDER = 8     # largest correlation order the programs encode
 
 
 def pool_map(aspec):
@@ -150,9 +152,9 @@ def eval_terms(terms, ct):
 class RunProgram:
     """Flat encoding of a set of polynomials ("tasks").
 
-    run  = (depth, jstart, f1, f2, f3, nleaf, leaf_off): depth = prefix length
-           (0..3); jstart = first prefix level (1-based) that differs from the
-           previous run of the same task (depth+1 when none).
+    run  = (depth, jstart, nleaf, leaf_off, f1, ..., f7) (RUN_W words): depth =
+           prefix length (0..7); jstart = first prefix level (1-based) that
+           differs from the previous run of the same task (depth+1 when none).
     leaf = device factor id + symbolic coefficient terms.
     task = (run_begin, run_end, out_slot) + constant terms; out_slot -1 is
            the (real part of the) energy, otherwise a G slot.
@@ -176,8 +178,8 @@ class RunProgram:
             if len(mono) == 0:
                 const_terms.extend(terms)
                 continue
-            if len(mono) > 4:
-                raise ValueError("correlation order above 4 is not supported by the device programs")
+            if len(mono) > MAX_ORDER:
+                raise ValueError("correlation order above 8 is not supported by the device programs")
             groups[mono[:-1]].append((mono[-1], terms))
         r0 = len(self.runs)
         prev = ()
@@ -187,8 +189,8 @@ class RunProgram:
             j = 1
             while j <= d and j <= len(prev) and pre[j - 1] == prev[j - 1]:
                 j += 1
-            f = [self.dev(x) for x in pre] + [0] * (3 - d)
-            self.runs.append((d, j, f[0], f[1], f[2], len(leaves), len(self.leaf_fid)))
+            f = [self.dev(x) for x in pre] + [0] * (RUN_W - 4 - d)
+            self.runs.append((d, j, len(leaves), len(self.leaf_fid), *f))
             for fl, terms in leaves:
                 self.leaf_fid.append(self.dev(fl))
                 self.leaf_terms.append(terms)
@@ -199,7 +201,7 @@ class RunProgram:
 
     def cost(self, t, leaf_w=1.0, run_w=2.0):
         r0, r1 = self.tasks[t][0], self.tasks[t][1]
-        return leaf_w * sum(self.runs[r][5] for r in range(r0, r1)) + run_w * (r1 - r0)
+        return leaf_w * sum(self.runs[r][2] for r in range(r0, r1)) + run_w * (r1 - r0)
 
 
 def split_poly(poly, nparts):

@@ -1,0 +1,349 @@
+// libacehip.so, 1p basis and pooling (product_1pbasis.jl:99-134): per
+// neighbour the radial basis (orthpolys.jl:253-263 after
+// distancetransforms.jl:73) and the m >= 0 complex spherical harmonics
+// (sphericalharmonics.jl:43-51,167-197,379-403), pooled over the neighbours
+// of each environment into A+[k][env]; and the directional derivative of the
+// pooled values along per-neighbour weights (adjoint_EVAL_D).
+#include "ace_common.hpp"
+
+// k_pool: A+[k][env].  EPW environments per wave, one per group of
+// LPE = 64 / EPW lanes (EPW = 2 when npool <= 32, so that small bases keep
+// the lanes busy).  A group stages its environment's neighbours POOL_CH at a
+// time in its own LDS region ([slot][R_n | Y_lm | species]); its lanes then
+// turn into pooled functions k = hl + LPE p (p < KP) and sum over the staged
+// neighbours in order, holding the sums in registers across chunks.
+template <int L, int EPW, int KP>
+__global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
+                                              const int32_t* __restrict__ off,
+                                              const double* __restrict__ xyz, int64_t nnbr,
+                                              const int8_t* __restrict__ species,
+                                              const int32_t* __restrict__ pool, int npool,
+                                              double2* __restrict__ Aplus, int64_t estride,
+                                              int32_t* __restrict__ nbr_env) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    constexpr int LPE = 64 / EPW;                 // lanes per environment
+    static_assert(EPW == 1 || KP == 1, "two environments per wave take one pooled function per lane");
+    extern __shared__ double smem[];  // [EPW][POOL_CH][rstride]
+    const int lane = threadIdx.x;
+    const int hl = lane % LPE;
+    const int nr = rp.nr;
+    const int rstride = nr + 2 * NP + 1;  // doubles per neighbour slot
+    const int64_t nblk = (nenv + EPW - 1) / EPW;
+    const int64_t env = xcd_env(blockIdx.x, nblk) * EPW + lane / LPE;
+    const bool valid = env < nenv;
+    const int j0 = valid ? off[env] : 0, j1 = valid ? off[env + 1] : 0;
+    double* const sgrp = smem + (lane / LPE) * POOL_CH * rstride;
+    // this lane's pooled functions k = hl + LPE p: slot offsets of R_n and Y_lm
+    int kr[KP], ky[KP], kmu[KP];
+    double ar[KP], ai[KP];
+#pragma unroll
+    for (int p = 0; p < KP; ++p) {
+        const int k = hl + LPE * p;
+        kr[p] = ky[p] = 0;
+        kmu[p] = -2;  // -2: no pooled function in this pass
+        if (k < npool) {
+            kmu[p] = pool[4 * k];
+            kr[p] = pool[4 * k + 1] - 1;
+            ky[p] = nr + 2 * idxp(pool[4 * k + 2], pool[4 * k + 3]);
+        }
+        ar[p] = ai[p] = 0.0;
+    }
+    for (int jb = j0; jb < j1; jb += POOL_CH) {
+        const int nch = min(POOL_CH, j1 - jb);
+        if (hl < nch) {
+            const int j = jb + hl;
+            if (nbr_env) nbr_env[j] = (int32_t)env;  // neighbour -> environment, for the flat force kernel
+            const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+            double* slot = sgrp + hl * rstride;
+            Sph S = cart2spher(x, y, z);
+            // radial basis
+            double u = (1.0 + rp.r0) / (1.0 + S.r);
+            double t = ipow(u, rp.tp);
+            double p1 = rp.A[0] * fcut(rp, t);
+            slot[0] = p1;
+            double p0 = 0.0;
+            if (nr > 1) {
+                p0 = p1;
+                p1 = (rp.A[1] * t + rp.B[1]) * p0;
+                slot[1] = p1;
+            }
+            for (int n = 2; n < nr; ++n) {
+                double p2 = (rp.A[n] * t + rp.B[n]) * p1 + rp.C[n] * p0;
+                p0 = p1;
+                p1 = p2;
+                slot[n] = p2;
+            }
+            // Y_lm = e^{i m phi}/sqrt2 P_lm, m-outer: P_mm from P_{m-1,m-1}, then
+            // the l recursion at fixed m (the operations of alp_eval, in an
+            // order that keeps two degrees live), written straight to LDS
+            double* ys = slot + nr;
+            const double isq2 = 0.7071067811865475;
+            double epr = isq2, epi = 0.0;
+            double Pmm = 0.3989422804014327;  // sqrt(1/(2 pi))
+#pragma unroll
+            for (int m = 0; m <= L; ++m) {
+                if (m == 1) {
+                    Pmm = -1.224744871391589 * S.sth * Pmm;
+                } else if (m >= 2) {
+                    Pmm = -sqrt(1.0 + 0.5 / m) * S.sth * Pmm;
+                }
+                if (m > 0) {
+                    const double nr_ = epr * S.cphi - epi * S.sphi;
+                    const double ni_ = epr * S.sphi + epi * S.cphi;
+                    epr = nr_;
+                    epi = ni_;
+                }
+                double Pa = 0.0, Pb = Pmm;
+#pragma unroll
+                for (int l = m; l <= L; ++l) {
+                    double Pl;
+                    if (l == m) {
+                        Pl = Pmm;
+                    } else if (l == m + 1) {
+                        Pl = m == 0 ? S.cth * 1.7320508075688772 * 0.3989422804014327
+                                    : S.cth * sqrt(2.0 * m + 3.0) * Pmm;
+                    } else {
+                        const int i = idxp(l, m);
+                        Pl = rp.alpA[i] * (S.cth * Pb + rp.alpB[i] * Pa);
+                    }
+                    if (l > m) {
+                        Pa = Pb;
+                        Pb = Pl;
+                    }
+                    ys[2 * idxp(l, m)] = (m == 0 ? isq2 : epr) * Pl;
+                    ys[2 * idxp(l, m) + 1] = m == 0 ? 0.0 : epi * Pl;
+                }
+            }
+            slot[nr + 2 * NP] = species ? (double)species[j] : -1.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // lanes over pooled functions, neighbours summed in order
+        for (int jj = 0; jj < nch; ++jj) {
+            const double* slot = sgrp + jj * rstride;
+            const int smu = species ? (int)slot[nr + 2 * NP] : -1;
+#pragma unroll
+            for (int p = 0; p < KP; ++p) {
+                if (kmu[p] == -1 || kmu[p] == smu) {
+                    const double R = slot[kr[p]];
+                    ar[p] += R * slot[ky[p]];
+                    ai[p] += R * slot[ky[p] + 1];
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int p = 0; p < KP; ++p)
+        if (kmu[p] != -2) Aplus[(int64_t)(hl + LPE * p) * estride + env] = make_double2(ar[p], ai[p]);
+}
+
+
+// k_poolw: dAw+[k][env] = sum_j w_j . grad_j phi_k over the pooled (m >= 0)
+// 1p functions: the directional derivative of A+ along per-neighbour weights
+// w[3][nnbr] (evaluator.jl:213-223, dAw = sum_j contract(w_j, dA[k, j]),
+// restated on A+; m < 0 follows by conjugation since w is real).  One wave
+// per environment; neighbours staged POOL_CH at a time in LDS as
+// [R_n | w.grad R_n | Y_lm | w.grad Y_lm | species] (radial and Ylm
+// gradients as in k_dphi), then lanes = pooled functions sum over them in
+// order: dAw+[k] += w.grad R_n Y_lm + R_n w.grad Y_lm.
+template <int L, int KP>
+__global__ __launch_bounds__(64) void k_poolw(RadialParams rp, int64_t nenv, const int32_t* __restrict__ off,
+                                              const double* __restrict__ xyz, const double* __restrict__ w,
+                                              int64_t nnbr, const int8_t* __restrict__ species,
+                                              const int32_t* __restrict__ pool, int npool,
+                                              double2* __restrict__ dAw, int64_t estride) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    const int64_t env = blockIdx.x;
+    const int nr = rp.nr;
+    const int rstride = 2 * nr + 4 * NP + 1;
+    if (env >= nenv) return;
+    const int j0 = off[env], j1 = off[env + 1];
+    int kr[KP], ky[KP], kmu[KP];
+    double ar[KP], ai[KP];
+#pragma unroll
+    for (int p = 0; p < KP; ++p) {
+        const int k = lane + 64 * p;
+        kr[p] = ky[p] = 0;
+        kmu[p] = -2;
+        if (k < npool) {
+            kmu[p] = pool[4 * k];
+            kr[p] = pool[4 * k + 1] - 1;
+            ky[p] = 2 * nr + 2 * idxp(pool[4 * k + 2], pool[4 * k + 3]);
+        }
+        ar[p] = ai[p] = 0.0;
+    }
+    for (int jb = j0; jb < j1; jb += POOL_CH) {
+        const int nch = min(POOL_CH, j1 - jb);
+        if (lane < nch) {
+            const int j = jb + lane;
+            const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
+            const double wx = w[j], wy = w[nnbr + j], wz = w[2 * nnbr + j];
+            double* slot = smem + lane * rstride;
+            const Sph S = cart2spher(x, y, z);
+            double t, dtdr;
+            transform_ed(rp, S.r, t, dtdr);
+            const double wr = (wx * x + wy * y + wz * z) / S.r;  // w . rhat
+            // R_n and dR_n/dt by the recursion of k_dphi; slot holds R_n and
+            // w . grad R_n = dR_n/dt dt/dr (w . rhat)
+            double Rm2 = 0.0, dRm2 = 0.0;
+            double Rm1 = rp.A[0] * fcut(rp, t), dRm1 = rp.A[0] * fcut_d(rp, t);
+            slot[0] = Rm1;
+            slot[nr] = dRm1 * dtdr * wr;
+            for (int n = 1; n < nr; ++n) {
+                const double al = rp.A[n] * t + rp.B[n];
+                double Rn = al * Rm1, dRn = al * dRm1 + rp.A[n] * Rm1;
+                if (n >= 2) {
+                    Rn += rp.C[n] * Rm2;
+                    dRn += rp.C[n] * dRm2;
+                }
+                Rm2 = Rm1;
+                dRm2 = dRm1;
+                Rm1 = Rn;
+                dRm1 = dRn;
+                slot[n] = Rn;
+                slot[nr + n] = dRn * dtdr * wr;
+            }
+            double P[NP], dP[NP];
+            alp_eval_ed<L>(rp, S, P, dP);
+            const double rinv = 1.0 / (S.r + 2.220446049250313e-16);
+            // w . grad = (fphi-part) + (ftheta-part) with the Cartesian rows of k_dphi
+            const double gph = (-S.sphi * wx + S.cphi * wy) * rinv;
+            const double gth = (S.cphi * S.cth * wx + S.sphi * S.cth * wy - S.sth * wz) * rinv;
+            double* ys = slot + 2 * nr;
+            const double isq2 = 0.7071067811865475;
+            for (int l = 0; l <= L; ++l) {
+                const int i = idxp(l, 0);
+                ys[2 * i] = P[i] * isq2;
+                ys[2 * i + 1] = 0.0;
+                ys[2 * NP + 2 * i] = gth * dP[i] * isq2;
+                ys[2 * NP + 2 * i + 1] = 0.0;
+            }
+            double epr = isq2, epi = 0.0;
+            for (int m = 1; m <= L; ++m) {
+                const double a = epr * S.cphi - epi * S.sphi;
+                const double b = epr * S.sphi + epi * S.cphi;
+                epr = a;
+                epi = b;
+                for (int l = m; l <= L; ++l) {
+                    const int i = idxp(l, m);
+                    const double pds = P[i], dpt = dP[i];
+                    ys[2 * i] = epr * pds * S.sth;
+                    ys[2 * i + 1] = epi * pds * S.sth;
+                    // fphi = i m ep P~, ftheta = ep dP~
+                    ys[2 * NP + 2 * i] = gph * (-m * epi * pds) + gth * (epr * dpt);
+                    ys[2 * NP + 2 * i + 1] = gph * (m * epr * pds) + gth * (epi * dpt);
+                }
+            }
+            slot[2 * nr + 4 * NP] = species ? (double)species[j] : -1.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int jj = 0; jj < nch; ++jj) {
+            const double* slot = smem + jj * rstride;
+            const int smu = species ? (int)slot[2 * nr + 4 * NP] : -1;
+#pragma unroll
+            for (int p = 0; p < KP; ++p) {
+                if (kmu[p] == -1 || kmu[p] == smu) {
+                    const double Rv = slot[kr[p]], wdR = slot[nr + kr[p]];
+                    const double Yr = slot[ky[p]], Yi = slot[ky[p] + 1];
+                    const double dYr = slot[2 * NP + ky[p]], dYi = slot[2 * NP + ky[p] + 1];
+                    ar[p] += wdR * Yr + Rv * dYr;
+                    ai[p] += wdR * Yi + Rv * dYi;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+#pragma unroll
+    for (int p = 0; p < KP; ++p)
+        if (kmu[p] != -2) dAw[(int64_t)(lane + 64 * p) * estride + env] = make_double2(ar[p], ai[p]);
+}
+
+template <int L>
+static int launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
+                       const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride, bool nbr_map) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    // two environments per wave when a half-wave covers the pooled functions
+    const int epw = ctx->npool <= 32 ? 2 : 1;
+    const size_t lds = (size_t)epw * POOL_CH * (ctx->nr + 2 * NP + 1) * sizeof(double);
+    int32_t* nbr_env = nullptr;
+    if (nbr_map && nnbr > 0) {
+        if (ctx->nbr_env.ensure(nnbr)) return -1;
+        nbr_env = ctx->nbr_env.p;
+    }
+    const dim3 grid((unsigned)((nenv + epw - 1) / epw));
+    const int8_t* sp = ctx->nspecies > 0 ? species : nullptr;
+    // pooled functions per lane: 1 (two environments per wave), 4 or 8
+    if (epw == 2) {
+        set_func_lds((const void*)k_pool<L, 2, 1>, lds);
+        hipLaunchKernelGGL((k_pool<L, 2, 1>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
+                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+    } else if (ctx->npool <= 256) {
+        set_func_lds((const void*)k_pool<L, 1, 4>, lds);
+        hipLaunchKernelGGL((k_pool<L, 1, 4>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
+                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+    } else {
+        if (ctx->npool > 512) return set_err("pooled 1p basis too large (npool > 512)");
+        set_func_lds((const void*)k_pool<L, 1, 8>, lds);
+        hipLaunchKernelGGL((k_pool<L, 1, 8>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
+                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <int L>
+static int launch_poolw(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
+                         const double* w, int64_t nnbr, const int8_t* species, int64_t estride) {
+    constexpr int NP = (L + 1) * (L + 2) / 2;
+    const size_t lds = (size_t)POOL_CH * (2 * ctx->nr + 4 * NP + 1) * sizeof(double);
+    const int8_t* sp = ctx->nspecies > 0 ? species : nullptr;
+    if (ctx->npool <= 256) {
+        set_func_lds((const void*)k_poolw<L, 4>, lds);
+        hipLaunchKernelGGL((k_poolw<L, 4>), dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx), nenv, off,
+                           xyz, w, nnbr, sp, ctx->pool.p, ctx->npool, ctx->Wplus.p, estride);
+    } else {
+        if (ctx->npool > 512) return set_err("pooled 1p basis too large (npool > 512)");
+        set_func_lds((const void*)k_poolw<L, 8>, lds);
+        hipLaunchKernelGGL((k_poolw<L, 8>), dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx), nenv, off,
+                           xyz, w, nnbr, sp, ctx->pool.p, ctx->npool, ctx->Wplus.p, estride);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_pool_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
+                    int64_t nnbr, const int8_t* species, int64_t estride, bool nbr_map) {
+    switch (ctx->L) {
+        case 0: return launch_pool<0>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 1: return launch_pool<1>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 2: return launch_pool<2>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 3: return launch_pool<3>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 4: return launch_pool<4>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 5: return launch_pool<5>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 6: return launch_pool<6>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 7: return launch_pool<7>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        default: return launch_pool<8>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+    }
+}
+
+int launch_poolw_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
+                     const double* w, int64_t nnbr, const int8_t* species, int64_t estride) {
+    switch (ctx->L) {
+        case 0: return launch_poolw<0>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 1: return launch_poolw<1>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 2: return launch_poolw<2>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 3: return launch_poolw<3>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 4: return launch_poolw<4>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 5: return launch_poolw<5>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 6: return launch_poolw<6>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 7: return launch_poolw<7>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        default: return launch_poolw<8>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+    }
+}

@@ -27,12 +27,12 @@ inline std::string validate_pack(const ace_pack* pk) {
     if (pk->prop != ACE_PROP_INVARIANT && pk->prop != ACE_PROP_EUCLIDEAN_VECTOR) return "unknown property";
     if (pk->L < 0 || pk->L > 8) return "L must be in [0, 8]";
     if (pk->nr < 1 || pk->nr > 32) return "nr must be in [1, 32]";
-    if (pk->npool < 1 || pk->npool > 256) return "npool out of range (the pooling kernel holds at most 256 pooled functions)";
+    if (pk->npool < 1 || pk->npool > 512) return "npool out of range (at most 512 pooled functions)";
     if (pk->nspecies < 0 || pk->nspecies > 127) return "nspecies out of range";
     if (pk->tp < 1 || pk->tp > 16) return "transform power out of range";
     if (pk->pl < 0 || pk->pr < 0 || pk->pl > 16 || pk->pr > 16) return "envelope powers out of range";
     if (pk->nA < 0 || pk->nAA < 0 || pk->nB < 0 || pk->a2b_nnz < 0) return "negative table size";
-    if (pk->maxorder < 1 || pk->maxorder > 4) return "maxorder must be in [1, 4]";
+    if (pk->maxorder < 1 || pk->maxorder > ACE_MAX_ORDER) return "maxorder must be in [1, 8]";
     if (!pk->rA || !pk->rB || !pk->rC || !pk->pool) return "null radial/pool table";
     for (int k = 0; k < pk->npool; ++k) {
         const int mu = pk->pool[4 * k], n = pk->pool[4 * k + 1], l = pk->pool[4 * k + 2],
@@ -57,12 +57,12 @@ inline std::string validate_pack(const ace_pack* pk) {
     if (pk->nruns < 0 || pk->nleaves < 0 || pk->ntasks < 0 || pk->nterms < 0 || pk->ngslots < 0)
         return "negative program size";
     for (int r = 0; r < pk->nruns; ++r) {
-        const int32_t* run = pk->runs + 7 * (size_t)r;
-        if (run[0] < 0 || run[0] > 3 || run[1] < 1 || run[5] < 0 || run[6] < 0 ||
-            (int64_t)run[6] + run[5] > pk->nleaves)
+        const int32_t* run = pk->runs + ACE_RUN_W * (size_t)r;
+        if (run[0] < 0 || run[0] > ACE_MAX_ORDER - 1 || run[0] > pk->maxorder - 1 || run[1] < 1 ||
+            run[1] > run[0] + 1 || run[2] < 1 || run[3] < 0 || (int64_t)run[3] + run[2] > pk->nleaves)
             return "run out of range";
         for (int q = 0; q < run[0]; ++q)
-            if (run[2 + q] < 0 || (run[2 + q] & 0x7fff) >= pk->npool) return "run factor out of range";
+            if (run[4 + q] < 0 || (run[4 + q] & 0x7fff) >= pk->npool) return "run factor out of range";
     }
     for (int i = 0; i < pk->nleaves; ++i)
         if (pk->leaf_fid[i] < 0 || (pk->leaf_fid[i] & 0x7fff) >= pk->npool) return "leaf factor out of range";

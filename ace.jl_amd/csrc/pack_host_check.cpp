@@ -1,7 +1,9 @@
-// CPU test shim over pack_host.hpp: the host-side pack logic of libacehip.so
-// built without HIP (tests build it with the address sanitizer).
+// CPU test shim over pack_host.hpp and compile_host.hpp: the host-side pack
+// logic and the program compiler of libacehip.so built without HIP (tests
+// build it with the address sanitizer, and plain for the larger bases).
 #include <cstring>
 
+#include "compile_host.hpp"
 #include "nbrlist_host.hpp"
 #include "pack_host.hpp"
 
@@ -59,4 +61,48 @@ extern "C" int acehost_nbr_grid(int64_t natoms, const double* pos, const double*
     std::memcpy(wrap, g.wrap.data(), g.wrap.size() * sizeof(int32_t));
     std::memcpy(pw, g.p.data(), g.p.size() * sizeof(double));
     return 0;
+}
+
+// The program compiler (compile_host.hpp): compiles an ace_jl_basis, keeps the
+// tables, and hands out any of them by name for comparison with the Python
+// host's pack.  acehost_compiled_array returns the element count (copying
+// into out when it is not null) or -1 for an unknown name.
+extern "C" void* acehost_compile(const ace_jl_basis* b, char* msg, int len) {
+    auto* ps = new acehost::PackStore();
+    std::string e = acehost::compile_jl(b, *ps);
+    if (e.empty()) {
+        const ace_pack pk = ps->view();
+        e = acehost::validate_pack(&pk);
+    }
+    if (msg && len > 0) {
+        std::strncpy(msg, e.c_str(), (size_t)len - 1);
+        msg[len - 1] = 0;
+    }
+    if (!e.empty()) {
+        delete ps;
+        return nullptr;
+    }
+    return ps;
+}
+
+extern "C" void acehost_compiled_free(void* p) { delete static_cast<acehost::PackStore*>(p); }
+
+extern "C" int64_t acehost_compiled_array(void* p, const char* name, void* out) {
+    const auto* ps = static_cast<const acehost::PackStore*>(p);
+    auto put = [&](const auto& v) -> int64_t {
+        if (out && !v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+        return (int64_t)v.size();
+    };
+    const std::string n(name);
+#define ACE_ARR(F) \
+    if (n == #F) return put(ps->F);
+    ACE_ARR(rA) ACE_ARR(rB) ACE_ARR(rC) ACE_ARR(pool) ACE_ARR(aspec_fid) ACE_ARR(aspec_sgn) ACE_ARR(orders)
+    ACE_ARR(iAA2iA) ACE_ARR(a2b_rowptr) ACE_ARR(a2b_cols) ACE_ARR(a2b_vals) ACE_ARR(runs) ACE_ARR(leaf_fid)
+    ACE_ARR(leaf_term_ptr) ACE_ARR(tasks) ACE_ARR(task_term_ptr) ACE_ARR(term_row) ACE_ARR(term_scale)
+    ACE_ARR(term_conj) ACE_ARR(gslot_ptr) ACE_ARR(gslot_list) ACE_ARR(rec_hdr) ACE_ARR(rec_leaf) ACE_ARR(feat_rows)
+    ACE_ARR(feat_ptr) ACE_ARR(feat_f) ACE_ARR(feat_coef) ACE_ARR(fgrad_ptr) ACE_ARR(fgrad_feat) ACE_ARR(fgrad_f)
+    ACE_ARR(fgrad_coef) ACE_ARR(wave_cfg_W) ACE_ARR(wave_task_ptr) ACE_ARR(wave_task_list) ACE_ARR(wave_etask_ptr)
+    ACE_ARR(wave_etask_list)
+#undef ACE_ARR
+    return -1;
 }

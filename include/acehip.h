@@ -32,9 +32,12 @@
 extern "C" {
 #endif
 
-#define ACEHIP_ABI_VERSION 1
+#define ACEHIP_ABI_VERSION 2
 
 typedef struct ace_ctx ace_ctx;
+
+#define ACE_RUN_W 11      /* int32 words per run of the device programs */
+#define ACE_MAX_ORDER 8   /* largest correlation order of a basis */
 
 /* Property of the symmetric basis: Invariant (properties.jl:77-152) or
  * EuclideanVector (properties.jl:165-248). */
@@ -57,9 +60,10 @@ enum { ACE_PROP_INVARIANT = 0, ACE_PROP_EUCLIDEAN_VECTOR = 1 };
  * A2B map (SymmetricBasis.A2Bmap) in CSR: a2b_rowptr[nB+1], a2b_cols[nnz],
  *   a2b_vals[nnz * (1 | 6)] (real, or 3 complex for EuclideanVector).
  * Programs: run programs for P = sum c~ AA and its gradient polynomials
- *   G[k] (see ace.jl_amd/acehip/programs.py); leaf coefficients are linear in
- *   c~: leaf_terms[leaf] = (term_begin) into term arrays (term_row,
- *   term_scale, term_conj); task_terms likewise for task constants.
+ *   G[k] (compiled by ace_ctx_create_jl from the basis tables, or by the
+ *   Python host); leaf coefficients are linear in c~: leaf_terms[leaf] =
+ *   (term_begin) into term arrays (term_row, term_scale, term_conj);
+ *   task_terms likewise for task constants.
  */
 typedef struct ace_pack {
     int32_t abi_version;
@@ -87,7 +91,8 @@ typedef struct ace_pack {
     const double* a2b_vals;       /* [nnz * ncoef] */
     /* run programs */
     int32_t nruns;
-    const int32_t* runs;          /* [nruns][7] depth,jstart,f1,f2,f3,nleaf,leaf_off */
+    const int32_t* runs;          /* [nruns][ACE_RUN_W] depth, jstart, nleaf, leaf_off, prefix
+                                     factors f1..f7 (depth <= 7: correlation order <= 8) */
     int32_t nleaves;
     const int32_t* leaf_fid;      /* [nleaves] */
     const int32_t* leaf_term_ptr; /* [nleaves+1] */
@@ -102,9 +107,10 @@ typedef struct ace_pack {
     int32_t ngslots;              /* number of G output slots */
     const int32_t* gslot_ptr;     /* [npool+1] slots summed into G[k] */
     const int32_t* gslot_list;    /* [ngslots] */
-    /* the same programs as 64-byte records: 8 header dwords (dw0 = depth |
-     * jstart << 2 | nleaf << 5, dw1 = f1 | f2 << 16, dw2 = f3, dw3.. leaf
-     * factor ids two per dword) + 4 coefficients filled from rec_leaf */
+    /* the same programs as 64-byte records (Invariant, correlation order <= 4;
+     * nrec = 0 otherwise): 8 header dwords (dw0 = depth | jstart << 2 |
+     * nleaf << 5, dw1 = f1 | f2 << 16, dw2 = f3, dw3.. leaf factor ids two per
+     * dword) + 4 coefficients filled from rec_leaf */
     int32_t nrec;
     const uint32_t* rec_hdr;      /* [nrec][8] */
     const int32_t* rec_leaf;      /* [nrec][4] leaf index per slot, -1 empty */
@@ -137,6 +143,67 @@ typedef struct ace_pack {
 int ace_ctx_create(const ace_pack* pack, int device, const double* c, int64_t nB,
                    ace_ctx** out);
 
+/*
+ * A basis as ACE.jl holds it: the fields of SymmetricBasis, PIBasis,
+ * Product1pBasis and the radial OrthPolyBasis in their Julia memory layout
+ * (Int = int64, 1-based indices, column-major Matrix, SparseMatrixCSC).  A
+ * Julia caller passes pointers to the objects' own arrays; nothing derived.
+ *
+ *  - radial basis: OrthPolyBasis fields pl, tl, pr, tr, A, B, C
+ *    (orthpolys.jl:82-99) and the transform t = ((1 + r0)/(1 + r))^tp
+ *    (PolyTransform, distancetransforms.jl:16,73);
+ *  - maxL: the degree bound of the SHBasis (Ylm component);
+ *  - ncategories: length of the Categorical1pBasis (0 when absent);
+ *  - Product1pBasis.indices (product_1pbasis.jl:5-8), Vector{NTuple{ncomp,
+ *    Int}} -> indices[nA][ncomp], with comp_kind[c] the kind of bases[c]:
+ *    ACE_COMP_CATEGORICAL (category index), ACE_COMP_RN (n), ACE_COMP_YLM
+ *    (index_y(l, m) = m + l + l^2 + 1, sphericalharmonics.jl:102);
+ *  - PIBasis.spec (pibasis.jl:10-13): orders[nAA], iAA2iA column-major
+ *    [maxorder][nAA] (entry (i, t) at iAA2iA[(t - 1) * nAA + i - 1]), 0 padding;
+ *  - SymmetricBasis.A2Bmap (symmbasis.jl:33-38), nB x nAA CSC: colptr[nAA+1],
+ *    rowval[nnz], nzval[nnz] (Invariant{Float64}) or nzval[nnz][6]
+ *    (EuclideanVector{ComplexF64}: 3 complex components, re/im interleaved).
+ */
+enum { ACE_COMP_CATEGORICAL = 0, ACE_COMP_RN = 1, ACE_COMP_YLM = 2 };
+
+typedef struct ace_jl_basis {
+    int32_t abi_version;
+    int32_t prop;                 /* ACE_PROP_INVARIANT / ACE_PROP_EUCLIDEAN_VECTOR */
+    int64_t nr;
+    const double* A;
+    const double* B;
+    const double* C;
+    int64_t pl, pr;
+    double tl, tr;
+    int64_t tp;
+    double r0;
+    int64_t maxL;
+    int64_t ncategories;
+    int64_t ncomp;
+    const int32_t* comp_kind;     /* [ncomp] */
+    int64_t nA;
+    const int64_t* indices;       /* [nA][ncomp] */
+    int64_t nAA, maxorder;
+    const int64_t* orders;        /* [nAA] */
+    const int64_t* iAA2iA;        /* [maxorder][nAA] */
+    int64_t nB;
+    const int64_t* colptr;        /* [nAA + 1] */
+    const int64_t* rowval;        /* [nnz] */
+    const double* nzval;          /* [nnz] or [nnz][6] */
+} ace_jl_basis;
+
+/* Context from ACE.jl's own tables: ProductEvaluator(basis, c)
+ * (evaluator.jl:30-31 via LinearACEModel(basis, c; evaluator),
+ * linearmodel.jl:44-56).  Compiles the device programs (pooled 1p table,
+ * product-DAG runs and records, adjoint polynomials, task lists) on the host
+ * and creates the context as ace_ctx_create does. */
+int ace_ctx_create_jl(const ace_jl_basis* basis, int device, const double* c, int64_t nB, ace_ctx** out);
+
+/* Host-only: compiles the device tables of ``basis`` without a device and
+ * reports their sizes (npool, nruns, nleaves, ntasks, nrec, ngslots, nfeat
+ * in sizes[0..6]).  Returns the same errors as ace_ctx_create_jl. */
+int ace_compile_check(const ace_jl_basis* basis, int64_t* sizes);
+
 /* set_params!(model, c) (linearmodel.jl:67-71 -> evaluator.jl:48-57). */
 int ace_set_params(ace_ctx* ctx, const double* c, int64_t nB);
 
@@ -155,6 +222,12 @@ int ace_set_parts(ace_ctx* ctx, int parts);
  * n >= 0 uses n tiles and needs 2 or 3 parts per tile.  A tuning knob of
  * this implementation, like ace_set_parts; no counterpart in the reference. */
 int ace_set_tail_parts(ace_ctx* ctx, int ntail);
+
+/* The product-DAG launch of the last evaluation call: workgroups per tile,
+ * the number of tail tiles that ran one more part, and the kernel (0: the
+ * record kernel; 64, 32 or 16: the run interpreter with that many
+ * environments per tile).  Instrumentation; no counterpart in the reference. */
+int ace_last_dag_shape(ace_ctx* ctx, int* parts, int64_t* tail_tiles, int* kernel);
 
 /* Site energies, ProductEvaluator evaluate (evaluator.jl:121-147), batched.
  * E_out[nenv] (Invariant) or E_out[nenv][3] (EuclideanVector).  species

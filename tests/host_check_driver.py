@@ -44,17 +44,38 @@ for args in [(3, 6, 4, "Invariant", 0), (3, 6, 4, "EuclideanVector", 0), (3, 6, 
     assert lib.acehost_lookup(C.byref(s), kidx.ctypes.data_as(C.c_void_p), a4.ctypes.data_as(C.c_void_p)) == 0
     assert np.all(a4.reshape(-1, 4) == np.asarray(T["aspec"]))
     # corrupted packs are rejected with a message, never read out of bounds
-    for field, bad, expect in [("npool", 0, b"npool"), ("L", 9, b"L must"), ("maxorder", 5, b"maxorder"),
+    for field, bad, expect in [("npool", 0, b"npool"), ("L", 9, b"L must"), ("maxorder", 9, b"maxorder"),
                                ("ngslots", 0, b"task out of range")]:
         s2, keep2 = _lib.make_pack(pk)
         setattr(s2, field, bad)
         assert lib.acehost_validate(C.byref(s2), msg, 256) != 0 and expect in msg.value, (field, msg.value)
     bad_runs = pk.runs.copy()
-    bad_runs[0, 5] = 10 ** 6
+    bad_runs[0, 2] = 10 ** 6
     s3, keep3 = _lib.make_pack(pk)
     keep3.append(np.ascontiguousarray(bad_runs, dtype=np.int32))
     s3.runs = keep3[-1].ctypes.data_as(C.POINTER(C.c_int32))
     assert lib.acehost_validate(C.byref(s3), msg, 256) != 0 and b"run out of range" in msg.value
+# the program compiler (compile_host.hpp) under the sanitizers: the tables of
+# small bases equal the Python packer's
+lib.acehost_compile.restype = C.c_void_p
+lib.acehost_compile.argtypes = [C.POINTER(_lib.AceJlBasis), C.c_char_p, C.c_int]
+lib.acehost_compiled_array.restype = C.c_int64
+lib.acehost_compiled_array.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+lib.acehost_compiled_free.argtypes = [C.c_void_p]
+for args in [(3, 6, 4, "Invariant", 0), (3, 6, 3, "EuclideanVector", 2), (5, 5, 2, "Invariant", 0)]:
+    basis, T = basis_for(*args)
+    pk = Pack(T)
+    s, keep = _lib.make_jl_basis(T)
+    msg = C.create_string_buffer(256)
+    h = lib.acehost_compile(C.byref(s), msg, 256)
+    assert h, msg.value
+    for name, dt in [("runs", np.int32), ("leaf_fid", np.int32), ("term_scale", np.float64), ("tasks", np.int32),
+                     ("rec_hdr", np.uint32), ("wave_task_list", np.int32), ("fgrad_f", np.int32)]:
+        n = lib.acehost_compiled_array(h, name.encode(), None)
+        a = np.zeros(n, dtype=dt)
+        lib.acehost_compiled_array(h, name.encode(), a.ctypes.data_as(C.c_void_p))
+        assert np.array_equal(a, np.asarray(getattr(pk, name), dtype=dt).reshape(-1)), (args, name)
+    lib.acehost_compiled_free(h)
 print("host pack checks passed")
 # neighbour-list binning: bins are consistent and cover every pair within rcut
 import nbr_oracle as NO  # noqa: E402

@@ -2,8 +2,6 @@
 returns the oracle's energies, gradients, basis values and adjoints, and the
 binding's first and mixed second derivatives are checked against central
 finite differences of those energies."""
-from types import SimpleNamespace
-
 import numpy as np
 import pytest
 import torch
@@ -20,10 +18,11 @@ class _OracleEvaluator:
     def __init__(self, T, c, nB):
         self.T, self.c = T, np.array(c, dtype=np.float64)
         self.prop = 0 if T["prop"] == "Invariant" else 1
-        self.pack = SimpleNamespace(nB=nB)
+        self.nB = nB
 
     def set_params(self, c):
         self.c = np.array(c, dtype=np.float64)
+        self._autograd_c = None
 
     def _args(self, b):
         off = b.offsets.numpy()

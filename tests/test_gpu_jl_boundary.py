@@ -1,0 +1,106 @@
+"""The context built from ACE.jl's own tables (ace_ctx_create_jl: the library
+compiles the device programs from Product1pBasis.indices, PIBasisSpec and
+A2Bmap in CSC) against the one built from the Python packer's tables
+(ace_ctx_create): energies, gradients and basis values must be bit-identical.
+Bases beyond the record kernel (correlation order 5, more than 160 pooled
+values) run the run interpreter; they are checked against the oracle.
+
+Tolerances: energies / basis values within 1e-12 of the batch's largest
+value, forces within 1e-10 (as every parity test here)."""
+import numpy as np
+import pytest
+import torch
+
+import ace_oracle as O
+from acehip import synth
+from acehip.evaluator import EnvBatch, HIPEvaluator
+from helpers import basis_for, random_batch, rel_err
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-12
+F_TOL = 1e-10
+
+BOUNDARY_CASES = [
+    ("config1", (3, 6, 4), {}, 1, 300),
+    ("config2-small", (4, 12, 6), {}, 2, 200),
+    ("config4-small", (3, 10, 5), {"prop": "EuclideanVector", "species": 2}, 4, 70),
+    ("two-species", (3, 6, 3), {"species": 2}, None, 90),
+    ("order5", (5, 6, 3), {}, None, 130),
+]
+
+
+@pytest.mark.parametrize("name,args,kw,wid,nenv", BOUNDARY_CASES, ids=[c[0] for c in BOUNDARY_CASES])
+def test_jl_tables_bit_identical_to_python_pack(cuda, name, args, kw, wid, nenv):
+    print(f"[gpu] ace_ctx_create_jl vs ace_ctx_create: {name}", flush=True)
+    basis, T = basis_for(*args, **kw)
+    c = synth.coefficients(len(basis), 7)
+    ns = kw.get("species", 0)
+    if wid is not None:
+        off, R, sp = synth.workload_envs(wid, nenv=nenv)
+    else:
+        off, R, sp = random_batch(nenv, 31, nspecies=ns)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    out = {}
+    for compiler in ("native", "python"):
+        ev = HIPEvaluator(basis, c, device=0, waves=8, compiler=compiler)
+        E, dE = ev.energy_forces(b)
+        E1 = ev.energy(b)
+        B = ev.basis_B(b)
+        torch.cuda.synchronize()
+        out[compiler] = (E.clone(), dE.clone(), E1.clone(), B.clone())
+    for a, r in zip(out["native"], out["python"]):
+        assert torch.equal(a, r)
+    # and both against the oracle on the first environments
+    n = min(nenv, 12)
+    o = off[: n + 1]
+    Rs = R[: off[n]]
+    ss = None if sp is None else sp[: off[n]]
+    E, dE = out["native"][0].cpu().numpy(), out["native"][1].cpu().numpy()
+    assert rel_err(E[:n], O.energy(T, c, o, Rs, ss)) < E_TOL
+    g_ref = O.grad_config(T, c, o, Rs, ss)
+    g = dE.T[: off[n]] if T["prop"] == "Invariant" else dE.reshape(3, 3, -1).transpose(2, 0, 1)[: off[n]]
+    assert rel_err(g, g_ref) < F_TOL
+
+
+def test_order5_basis_vs_oracle(cuda):
+    """Correlation order 5 (degree 6, L 3): prefixes of four factors, run by
+    the interpreter with seven prefix levels; 200 environments = 4 tiles."""
+    print("[gpu] order-5 basis vs oracle", flush=True)
+    basis, T = basis_for(5, 6, 3)
+    assert max(T["orders"]) == 5
+    c = synth.coefficients(len(basis), 11)
+    off, R, _ = random_batch(200, 41, nmin=4, nmax=30)
+    ev = HIPEvaluator(basis, c, device=0, waves=8)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    torch.cuda.synchronize()
+    assert ev.last_dag_shape()[2] == 64
+    assert rel_err(E.cpu().numpy(), O.energy(T, c, off, R)) < E_TOL
+    assert rel_err(dE.cpu().numpy().T, O.grad_config(T, c, off, R)) < F_TOL
+    Bv = ev.basis_B(b).cpu().numpy()
+    B_ref = O.eval_B(T, O.eval_AA(T, O.eval_A(T, off, R)))
+    assert rel_err(Bv, B_ref) < E_TOL
+
+
+@pytest.mark.parametrize("deg,order,nenv", [(14, 3, 150), (14, 4, 70)])
+def test_invariant_above_160_pooled_values(cuda, deg, order, nenv):
+    """One species, degree 14, L 6: 195 pooled values, too many for a
+    64-environment LDS tile; the interpreter runs 32-environment tiles with two
+    task streams per wave.  Order 4 is the model the reference profiles
+    (profile/profile_linearmodel.jl:11-25)."""
+    print(f"[gpu] npool 195 invariant, order {order}, degree {deg}", flush=True)
+    basis, T = basis_for(order, deg, 6)
+    c = synth.coefficients(len(basis), 12)
+    off, R, _ = synth.workload_envs(2, nenv=nenv, seed=43)
+    ev = HIPEvaluator(basis, c, device=0, waves=8)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    E1 = ev.energy(b)
+    torch.cuda.synchronize()
+    assert ev.last_dag_shape()[2] == 32
+    assert rel_err(E.cpu().numpy(), O.energy(T, c, off, R)) < E_TOL
+    assert torch.equal(E, E1)
+    n = 40 if order == 4 else nenv
+    g_ref = O.grad_config(T, c, off[: n + 1], R[: off[n]])
+    assert rel_err(dE.cpu().numpy().T[: off[n]], g_ref) < F_TOL

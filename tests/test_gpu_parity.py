@@ -106,9 +106,10 @@ def test_tail_tiles_with_one_more_part(cuda, parts, tail):
 
 def test_tail_tiles_full_size(cuda):
     """At 100k environments (1,563 tiles) the automatic choice gives the tiles
-    of the grid's last round one more part.  Energies and gradients equal the
-    run without tail tiles, and on the environments of the last tile the
-    oracle."""
+    of the grid's last round one more part (checked through the launch the
+    library reports).  Energies and gradients equal the run without tail
+    tiles, and the oracle on the last tile's environments and on 48 more
+    spread over the whole batch (every XCD's range, full and tail tiles)."""
     print("[gpu] tail tiles at 100k environments", flush=True)
     basis, T = basis_for(4, 12, 6)
     c = synth.coefficients(len(basis), 2)
@@ -122,6 +123,12 @@ def test_tail_tiles_full_size(cuda):
         E, dE = ev.energy_forces(b)
         torch.cuda.synchronize()
         runs[(parts, tail)] = (E.cpu().numpy(), dE.cpu().numpy())
+        P, ntail, kern = ev.last_dag_shape()
+        assert kern == 0
+        if tail == -1:
+            assert P >= 2 and ntail > 0, (P, ntail)  # the automatic choice really runs tail tiles
+        if tail == 0:
+            assert ntail == 0
     E0, g0 = runs[(0, 0)]
     for key, (E, g) in runs.items():
         assert rel_err(E, E0) < 1e-14, key
@@ -132,6 +139,12 @@ def test_tail_tiles_full_size(cuda):
     Et, gt = runs[(3, 128)]
     assert rel_err(Et[e0:], O.energy(T, c, o, Rs)) < E_TOL
     assert rel_err(gt[:, off[e0]:].T, O.grad_config(T, c, o, Rs)) < F_TOL
+    Ea, ga = runs[(0, -1)]
+    for e in np.linspace(0, 100000 - 25, 48).astype(int):
+        oe = np.array([0, off[e + 1] - off[e]], dtype=np.int32)
+        Re = R[off[e]:off[e + 1]]
+        assert rel_err(Ea[e:e + 1], O.energy(T, c, oe, Re)) < E_TOL, e
+        assert rel_err(ga[:, off[e]:off[e + 1]].T, O.grad_config(T, c, oe, Re)) < F_TOL, e
 
 
 def test_tail_parts_rejects_bad_settings(cuda):
@@ -208,10 +221,12 @@ def test_config3_fcc_forces_fd(cuda):
 
 
 def test_vector_property(cuda):
-    print("[gpu] EuclideanVector order 3 degree 6", flush=True)
+    """EuclideanVector, order 3, degree 6: 64-environment tiles; 150
+    environments are two full tiles and a partial one."""
+    print("[gpu] EuclideanVector order 3 degree 6, 150 envs", flush=True)
     basis, T = basis_for(3, 6, 4, prop="EuclideanVector")
     c = synth.coefficients(len(basis), 4)
-    off, R, sp = random_batch(40, 11)
+    off, R, sp = random_batch(150, 11)
     _, E, dE = _run(basis, T, c, off, R)
     assert rel_err(E, O.energy(T, c, off, R)) < E_TOL
     g_ref = O.grad_config(T, c, off, R)              # [nbr, p, d]
@@ -314,13 +329,19 @@ def test_invalid_inputs_raise(cuda):
         ev.energy(EnvBatch(off2, R.T.copy(), sp, device="cuda:0"))
 
 
-def test_config4_two_species_vector(cuda):
-    print("[gpu] config4: 2 species, EuclideanVector, order 3, degree 10, L 5 (32-env tiles)", flush=True)
+@pytest.mark.parametrize("waves", [8, 16])
+def test_config4_two_species_vector(cuda, waves):
+    """Config 4 (2 species, EuclideanVector, order 3, degree 10, L 5): 220
+    pooled values, 32-environment tiles with two task streams per wave.
+    100 environments are three full tiles and a partial one; every energy
+    and every per-neighbour Jacobian is compared with the oracle."""
+    print(f"[gpu] config4: 100 envs, waves={waves} (32-env tiles)", flush=True)
     basis, T = basis_for(3, 10, 5, prop="EuclideanVector", species=2)
     c = synth.coefficients(len(basis), 4)
-    off, R, sp = synth.workload_envs(4, nenv=6)
-    _, E, dE = _run(basis, T, c, off, R, sp)
+    off, R, sp = synth.workload_envs(4, nenv=100)
+    ev, E, dE = _run(basis, T, c, off, R, sp, waves=waves)
+    assert ev.last_dag_shape()[2] == 32
     assert rel_err(E, O.energy(T, c, off, R, sp)) < E_TOL
-    g_ref = O.grad_config(T, c, off[:3], R[: off[2]], sp[: off[2]])
-    g = dE.reshape(3, 3, -1).transpose(2, 0, 1)[: off[2]]
+    g_ref = O.grad_config(T, c, off, R, sp)
+    g = dE.reshape(3, 3, -1).transpose(2, 0, 1)
     assert rel_err(g, g_ref) < F_TOL

@@ -26,7 +26,7 @@ def emulate(pk, Aplus, ct):
         acc = np.zeros((nenv, NC), dtype=complex) + np.asarray(tc[ti]).reshape(1, -1)
         P1 = P2 = P3 = np.ones(nenv, dtype=complex)
         for r in range(r0, r1):
-            d, j, f1, f2, f3, nl, lo = pk.runs[r]
+            d, j, nl, lo, f1, f2, f3 = pk.runs[r][:7]
             if j <= 1 and d >= 1:
                 P1 = fac(f1)
             if j <= 2 and d >= 2:
