@@ -447,7 +447,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         ctx->rowptr.upload(pk->a2b_rowptr, pk->nB + 1) ||
         ctx->cols.upload(pk->a2b_cols, pk->a2b_nnz) ||
         ctx->vals.upload(pk->a2b_vals, (size_t)pk->a2b_nnz * nc) ||
-        ctx->runs.upload(pk->runs, 7 * (size_t)pk->nruns) ||
+        ctx->runs.upload(pk->runs, ACE_RUN_W * (size_t)pk->nruns) ||
         ctx->leaf_fid.upload(pk->leaf_fid, pk->nleaves) ||
         ctx->d_tasks.upload(pk->tasks, 5 * (size_t)pk->ntasks) ||
         ctx->gslot_ptr.upload(pk->gslot_ptr, pk->npool + 1) ||
