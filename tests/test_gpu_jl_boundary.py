@@ -99,8 +99,10 @@ def test_invariant_above_160_pooled_values(cuda, deg, order, nenv):
     E1 = ev.energy(b)
     torch.cuda.synchronize()
     assert ev.last_dag_shape()[2] == 32
-    assert rel_err(E.cpu().numpy(), O.energy(T, c, off, R)) < E_TOL
-    assert torch.equal(E, E1)
+    E_ref = O.energy(T, c, off, R)
+    assert rel_err(E.cpu().numpy(), E_ref) < E_TOL
+    # the energy-only call sums the energy tasks over another wave assignment
+    assert rel_err(E1.cpu().numpy(), E_ref) < E_TOL
     n = 40 if order == 4 else nenv
     g_ref = O.grad_config(T, c, off[: n + 1], R[: off[n]])
     assert rel_err(dE.cpu().numpy().T[: off[n]], g_ref) < F_TOL
