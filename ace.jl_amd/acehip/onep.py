@@ -49,8 +49,11 @@ class Rn1pBasis:
     def isadmissible(self, b):
         return 1 <= b[self.nsym] <= len(self.J)
 
-    def degree(self, b):
-        return b[self.nsym] - 1
+    def degree(self, b, weight=None):
+        """n - 1 (Rn.jl:20); with a weight dict, weight[n-symbol] (n - 1)
+        (b1pcomponent.jl:145-147)."""
+        d = b[self.nsym] - 1
+        return d if weight is None else weight[self.nsym] * d
 
 
 class Ylm1pBasis:
@@ -76,8 +79,10 @@ class Ylm1pBasis:
         l, m = b[self.lsym], b[self.msym]
         return 0 <= l <= self.L and abs(m) <= l
 
-    def degree(self, b):
-        return b[self.lsym]
+    def degree(self, b, weight=None):
+        """l (Ylm.jl:24); with a weight dict, weight[l-symbol] l."""
+        d = b[self.lsym]
+        return d if weight is None else weight[self.lsym] * d
 
 
 class Categorical1pBasis:
@@ -104,8 +109,8 @@ class Categorical1pBasis:
     def isadmissible(self, b):
         return b[self.idxsym] in self.categories
 
-    def degree(self, b):
-        return 0
+    def degree(self, b, weight=None):
+        return 0  # discrete1pbasis.jl:123, with or without weights
 
     def val2i(self, val):
         """discrete1pbasis.jl:33-40: unknown categories are an error."""
@@ -163,8 +168,11 @@ class Product1pBasis:
     def isadmissible(self, b: dict):
         return all(B.isadmissible(b) for B in self.bases)
 
-    def degree(self, b: dict):
-        return sum(B.degree(b) for B in self.bases)
+    def degree(self, b: dict, weight=None):
+        """product_1pbasis.jl:328-331: the sum over the components."""
+        if weight is None:
+            return sum(B.degree(b) for B in self.bases)
+        return sum(B.degree(b, weight) for B in self.bases)
 
     def set_spec(self, spec):
         self.spec = [tuple(b) for b in spec]

@@ -38,11 +38,137 @@ class SimpleSparseBasis:
         return True
 
 
+class SparseBasis:
+    """``SparseBasis(; maxorder, p, weight, default_maxdeg | maxlevels)``
+    (basisselectors.jl:175-198, 133-158): the level of a 1p function is its
+    weighted degree (``weight[:n] (n - 1) + weight[:l] l``, b1pcomponent.jl:
+    145-147; a categorical component has degree 0, discrete1pbasis.jl:123),
+    the level of a product the p-norm of its factors' levels, and the bound
+    depends on the correlation order: ``maxlevels[ord]``, else
+    ``maxlevels["default"]``."""
+
+    def __init__(self, maxorder: int, p: float = 1.0, weight=None, default_maxdeg=None, maxlevels=None):
+        if (default_maxdeg is None) == (maxlevels is None):
+            raise ValueError("SparseBasis: give exactly one of default_maxdeg and maxlevels")
+        self.maxorder = int(maxorder)
+        self.p = float(p)
+        self.weight = {"l": 1.0, "n": 1.0} if weight is None else {k: float(v) for k, v in weight.items()}
+        self.maxlevels = {"default": float(default_maxdeg)} if maxlevels is None else \
+            {k: float(v) for k, v in maxlevels.items()}
+        # maxlevel(Bsel, basis): the largest bound over orders 1..maxorder (the 1p cut)
+        self.maxlevel = max(self.maxlevel_order(o) for o in range(1, self.maxorder + 1))
+
+    def maxlevel_order(self, order: int) -> float:
+        return self.maxlevels.get(order, self.maxlevels["default"])
+
+    def level1(self, b: dict, B1p) -> float:
+        return B1p.degree(b, self.weight)
+
+    def filter1(self, b: dict, B1p) -> bool:
+        return True
+
+    def level(self, bb, B1p) -> float:
+        return 0.0 if len(bb) == 0 else pnorm([self.level1(b, B1p) for b in bb], self.p)
+
+    def filter(self, bb, B1p) -> bool:
+        return True
+
+
+def pnorm(x, p: float) -> float:
+    """``norm(x, p)`` of non-negative levels as Julia's LinearAlgebra computes
+    it: a running sum for p = 1, the maximum for p = Inf, else the sum of
+    powers of x / max(x), rooted and rescaled by max(x)."""
+    x = [float(v) for v in x]
+    if not x:
+        return 0.0
+    if p == 1.0:
+        s = 0.0
+        for v in x:
+            s += abs(v)
+        return s
+    m = max(abs(v) for v in x)
+    if p == float("inf") or m == 0.0:
+        return m
+    s = 0.0
+    for v in x:
+        s += (abs(v) / m) ** p
+    return m * s ** (1.0 / p)
+
+
 class NoConstant:
     """basisselectors.jl:91-94."""
 
     def __call__(self, bb):
         return len(bb) > 0
+
+
+def gensparse_levels(NU: int, lev, maxlevel_of, p: float = 1.0):
+    """All tuples of ``gensparse_ordered`` for the downset of a level that is
+    the p-norm of 1p levels ``lev`` (non-decreasing in the 1p index) under a
+    bound ``maxlevel_of(order)``; int32 [ntuples, NU], 1-based, 0 = empty, in
+    the traversal's order, without the per-tuple filter.
+
+    For such a level the traversal of sparsegrids.jl:85-148 visits, order by
+    order (every order-k tuple precedes the order-(k+1) ones, since the empty
+    slots lead), exactly the admissible non-decreasing tuples in
+    lexicographic order: within a row the level grows with the last index,
+    so the row ends at its first inadmissible tuple and nothing admissible is
+    skipped, and a row whose first tuple is inadmissible ends its parent row
+    for the same reason.  Each order is therefore built by extending the
+    admissible prefixes with contiguous index ranges; the final test
+    recomputes each tuple's level as ``pnorm`` does."""
+    lev = np.asarray(lev, dtype=np.float64)
+    nA = len(lev)
+    if nA and np.any(np.diff(lev) < 0):
+        raise ValueError("gensparse_levels: 1p levels must be non-decreasing")
+    if np.any(lev < 0):
+        raise ValueError("gensparse_levels: 1p levels must be non-negative")
+    out = [np.zeros((1, NU), dtype=np.int32)]                 # the empty tuple (level 0)
+    pinf = p == float("inf")
+    lp = lev if (pinf or p == 1.0) else lev ** p
+    for k in range(1, NU + 1):
+        bound = maxlevel_of(k)
+        # the prefix search uses a slightly widened bound; the exact test follows
+        bp = (bound if (pinf or p == 1.0) else bound ** p) * (1 + 1e-12) + 1e-12
+        T = np.arange(nA, dtype=np.int32)[:, None]
+        acc = lp.copy()
+        keep = acc <= bp
+        T, acc = T[keep], acc[keep]
+        for _ in range(1, k):
+            last = T[:, -1].astype(np.int64)
+            if pinf:
+                hi = np.full(len(T), np.searchsorted(lp, bp, side="right"), dtype=np.int64)
+            else:
+                hi = np.searchsorted(lp, bp - acc, side="right").astype(np.int64)
+            cnt = np.maximum(hi - last, 0)
+            rows = np.repeat(np.arange(len(T)), cnt)
+            start = np.repeat(np.cumsum(cnt) - cnt, cnt)
+            nxt = (last[rows] + (np.arange(len(rows)) - start)).astype(np.int32)
+            acc = np.maximum(acc[rows], lp[nxt]) if pinf else acc[rows] + lp[nxt]
+            T = np.concatenate([T[rows], nxt[:, None]], axis=1)
+        if len(T) == 0:
+            continue
+        # exact level of every tuple (left to right, as pnorm)
+        L = lev[T]
+        if p == 1.0:
+            lv = np.zeros(len(T))
+            for j in range(k):
+                lv = lv + np.abs(L[:, j])
+        else:
+            m = np.abs(L).max(axis=1)
+            if pinf:
+                lv = m
+            else:
+                sm = np.zeros(len(T))
+                safe = np.where(m > 0, m, 1.0)
+                for j in range(k):
+                    sm = sm + (np.abs(L[:, j]) / safe) ** p
+                lv = np.where(m > 0, m * sm ** (1.0 / p), 0.0)
+        T = T[lv <= bound]
+        full = np.zeros((len(T), NU), dtype=np.int32)
+        full[:, NU - k:] = T + 1
+        out.append(full)
+    return np.concatenate(out, axis=0)
 
 
 def gensparse_ordered(NU: int, maxvv: int, admissible, filt):
@@ -96,6 +222,15 @@ class PIBasisSpec:
         return tuple(int(x) for x in self.iAA2iA[i, : self.orders[i]])
 
     @classmethod
+    def from_array(cls, V):
+        """From gensparse tuples V [n, NU] (1-based, 0 = empty, ascending):
+        iAA2iA holds each reversed (descending), 0-based, -1 padded."""
+        V = np.asarray(V, dtype=np.int64)
+        orders = (V > 0).sum(axis=1)
+        iAA2iA = (V[:, ::-1] - 1).astype(np.int64)  # empty slots (0) -> -1, last after reversal
+        return cls(orders, iAA2iA)
+
+    @classmethod
     def from_tuples(cls, AAspec, NU):
         orders = np.zeros(len(AAspec), dtype=np.int64)
         iAA2iA = -np.ones((len(AAspec), NU), dtype=np.int64)
@@ -120,8 +255,11 @@ class PIBasisSpec:
         self.iAA2iA = out
 
 
-def build_pispec(B1p, Bsel, prop=None, filterfun=None, init1pbasis=True):
-    """pibasis.jl:35-87."""
+def build_pispec(B1p, Bsel, prop=None, filterfun=None, init1pbasis=True, fast=True):
+    """pibasis.jl:35-87.  ``fast``: enumerate with :func:`gensparse_levels`
+    and a vectorised filter when the selector and filters allow it (same
+    tuples, same order); otherwise, or with ``fast=False``, the literal
+    traversal :func:`gensparse_ordered` with per-tuple filters."""
     from .onep import init1pspec
 
     if init1pbasis:
@@ -132,11 +270,23 @@ def build_pispec(B1p, Bsel, prop=None, filterfun=None, init1pbasis=True):
     if any(levels[i] > levels[i + 1] for i in range(len(levels) - 1)):
         raise ValueError("PIBasisSpec: Aspec is not sorted by degree")
     deg = np.asarray([B1p.degree(b) for b in Adicts], dtype=np.float64)
+    fast_mask = _fast_filter(prop, filterfun, Bsel)
+    if fast and fast_mask is not None:
+        p = Bsel.p if isinstance(Bsel, SparseBasis) else 1.0
+        maxlev = Bsel.maxlevel_order if isinstance(Bsel, SparseBasis) else (lambda k: Bsel.maxlevel)
+        V = gensparse_levels(Bsel.maxorder, levels, maxlev, p)
+        ll = np.asarray([b.get("l", 0) for b in Adicts], dtype=np.int64)
+        mm = np.asarray([b.get("m", 0) for b in Adicts], dtype=np.int64)
+        V = V[fast_mask(V, ll, mm)]
+        return PIBasisSpec.from_array(V)
 
     def tup2b(vv):
         return [Adicts[v - 1] for v in vv if v != 0]
 
     def admissible(vv):
+        if isinstance(Bsel, SparseBasis):
+            bb = [v for v in vv if v != 0]
+            return pnorm([levels[v - 1] for v in bb], Bsel.p) <= Bsel.maxlevel_order(len(bb))
         return sum(deg[v - 1] for v in vv if v != 0) <= Bsel.maxlevel
 
     def filt(vv):
@@ -151,6 +301,39 @@ def build_pispec(B1p, Bsel, prop=None, filterfun=None, init1pbasis=True):
 
     AAspec = gensparse_ordered(Bsel.maxorder, len(Aspec), admissible, filt)
     return PIBasisSpec.from_tuples(AAspec, Bsel.maxorder)
+
+
+def _fast_filter(prop, filterfun, Bsel):
+    """The tuple filter of build_pispec as a vectorised mask over gensparse
+    tuples V (1-based, 0 = empty) when every part of it is known: no
+    filterfun or NoConstant, a selector without a filter of its own, and the
+    Invariant (properties.jl:115-125) or EuclideanVector (:189-203)
+    selection rules on the sums of l and m.  None otherwise."""
+    from .coupling import EuclideanVector, Invariant
+
+    if filterfun is not None and not isinstance(filterfun, NoConstant):
+        return None
+    if prop is not None and not isinstance(prop, (Invariant, EuclideanVector)):
+        return None
+    if type(Bsel) not in (SimpleSparseBasis, SparseBasis):
+        return None
+
+    def mask(V, ll, mm):
+        nz = V > 0
+        order = nz.sum(axis=1)
+        idx = np.where(nz, V - 1, 0)
+        suml = np.where(nz, ll[idx], 0).sum(axis=1)
+        summ = np.where(nz, mm[idx], 0).sum(axis=1)
+        keep = np.ones(len(V), dtype=bool)
+        if isinstance(filterfun, NoConstant):
+            keep &= order > 0
+        if isinstance(prop, Invariant):
+            keep &= (order <= 1) | ((suml % 2 == 0) & (summ == 0))
+        elif isinstance(prop, EuclideanVector):
+            keep &= (order == 1) | ((order >= 2) & (suml % 2 == 1) & (np.abs(summ) <= 1))
+        return keep
+
+    return mask
 
 
 class PIBasis:

@@ -111,3 +111,57 @@ def test_gensparse_order_matches_bruteforce():
     brute = [vv for vv in itertools.combinations_with_replacement(range(len(deg) + 1), 3) if adm(vv)]
     assert sorted(spec) == sorted(brute)
     assert spec == sorted(spec)          # ascending lexicographic enumeration
+
+
+def _b1p(maxdeg, L, species=0):
+    RnYlm = ah.RnYlm_1pbasis(maxdeg=maxdeg, maxL=L, r0=1.0, rcut=5.0)
+    if not species:
+        return RnYlm
+    cat = ah.Categorical1pBasis(list(range(species)), varsym="mu", idxsym="mu")
+    return ah.Product1pBasis((cat,) + RnYlm.bases)
+
+
+FAST_CASES = [
+    ("simple-inv", lambda: ah.SimpleSparseBasis(3, 6), 6, 4, 0, "Invariant", None),
+    ("simple-vec", lambda: ah.SimpleSparseBasis(3, 6), 6, 4, 0, "EuclideanVector", None),
+    ("simple-species", lambda: ah.SimpleSparseBasis(3, 6), 6, 3, 2, "Invariant", None),
+    ("simple-noconst", lambda: ah.SimpleSparseBasis(4, 7), 7, 4, 0, "Invariant", ah.NoConstant()),
+    ("simple-order5", lambda: ah.SimpleSparseBasis(5, 6), 6, 3, 0, "Invariant", None),
+    ("sparse-weights", lambda: ah.SparseBasis(3, weight={"n": 1.0, "l": 1.5}, default_maxdeg=8), 8, 5, 0,
+     "Invariant", None),
+    ("sparse-p2", lambda: ah.SparseBasis(3, p=2, default_maxdeg=6), 6, 4, 0, "Invariant", None),
+    ("sparse-pinf", lambda: ah.SparseBasis(3, p=float("inf"), default_maxdeg=4), 4, 3, 0, "Invariant", None),
+    # order-dependent bounds that are not monotone in the order
+    ("sparse-orders", lambda: ah.SparseBasis(4, maxlevels={1: 9, 2: 5, 3: 7, "default": 4}), 9, 4, 0,
+     "EuclideanVector", None),
+]
+
+
+@pytest.mark.parametrize("name,mk,maxdeg,L,species,prop,ff", FAST_CASES, ids=[c[0] for c in FAST_CASES])
+def test_fast_gensparse_equals_literal_traversal(name, mk, maxdeg, L, species, prop, ff):
+    """gensparse_levels + the vectorised filter give the tuples of the literal
+    traversal (sparsegrids.jl:85-148 restated in gensparse_ordered) in the same
+    order, for SimpleSparseBasis and SparseBasis (basisselectors.jl:175-198)
+    with weights, p-norms and order-dependent bounds."""
+    from acehip.pibasis import build_pispec
+    P = ah.Invariant() if prop == "Invariant" else ah.EuclideanVector(True)
+    Bsel = mk()
+    fast = build_pispec(_b1p(maxdeg, L, species), Bsel, prop=P, filterfun=ff, fast=True)
+    slow = build_pispec(_b1p(maxdeg, L, species), Bsel, prop=P, filterfun=ff, fast=False)
+    assert len(fast) > 10
+    assert np.array_equal(fast.orders, slow.orders)
+    assert np.array_equal(fast.iAA2iA, slow.iAA2iA)
+
+
+def test_sparse_basis_levels():
+    """SparseBasis: weighted 1p degree, p-norm product level, order bounds."""
+    from acehip.pibasis import pnorm
+    B1p = _b1p(6, 4)
+    Bsel = ah.SparseBasis(3, p=2, weight={"n": 2.0, "l": 1.0}, maxlevels={2: 5.0, "default": 6.0})
+    b = {"n": 3, "l": 2, "m": 1}
+    assert Bsel.level1(b, B1p) == 2.0 * 2 + 1.0 * 2
+    assert Bsel.level([b, b], B1p) == pytest.approx(np.sqrt(2) * 6)
+    assert Bsel.maxlevel_order(2) == 5.0 and Bsel.maxlevel_order(3) == 6.0 and Bsel.maxlevel == 6.0
+    assert pnorm([3.0, 4.0], 2.0) == 5.0 and pnorm([1.0, 2.0, 3.0], float("inf")) == 3.0
+    with pytest.raises(ValueError):
+        ah.SparseBasis(3)
