@@ -346,7 +346,7 @@ static int basis_A_internal(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
     const int64_t estride = ((nenv + 63) / 64) * 64;
     if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
     if (ctx->Afull.ensure((size_t)nenv * ctx->nA)) return -1;
-    if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, false)) return -1;
+    if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), false)) return -1;
     const int64_t n = nenv * ctx->nA;
     hipLaunchKernelGGL(k_expandA, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nenv, ctx->nA,
                        ctx->aspec_fid.p, ctx->aspec_sgn.p, ctx->Aplus.p, estride, ctx->Afull.p);
@@ -444,7 +444,7 @@ extern "C" int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* off, const
         const int fits = bt_fits(ctx, false);
         if (fits) {
             if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
-            if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, false)) return -1;
+            if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), false)) return -1;
             return launch_bt<false>(ctx, st, nenv, estride, B_out);
         }
     }
@@ -512,7 +512,7 @@ extern "C" int ace_adjoint_EVAL_D(ace_ctx* ctx, int64_t nenv, const int32_t* off
     if (bt_fits(ctx, true)) {
         if (ctx->Aplus.ensure((size_t)ctx->npool * estride) || ctx->Wplus.ensure((size_t)ctx->npool * estride))
             return -1;
-        if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, false) ||
+        if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), false) ||
             launch_poolw_any(ctx, st, nenv, off, xyz, w, nnbr, species, estride))
             return -1;
         return launch_bt<true>(ctx, st, nenv, estride, dB_out);

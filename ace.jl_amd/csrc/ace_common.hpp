@@ -282,6 +282,30 @@ struct ace_ctx {
     }
 };
 
+// Device scratch of one evaluation call: the pooled values A+
+// [npool][estride], the adjoint G [ngslots * NC][estride], the per-part
+// energy shares [parts][estride] and the neighbour -> environment map of the
+// flat force kernel [nnbr].  Either the context's own buffers or carved from
+// the allocation-free entry point's fixed scratch array.
+struct Workspace {
+    double2* Aplus = nullptr;
+    double2* Gbuf = nullptr;
+    double* Epart = nullptr;
+    int32_t* nbr_env = nullptr;
+    size_t nbr_cap = 0;  // entries nbr_env holds
+};
+
+// The context's own buffers (as sized by their last ensure).
+inline Workspace ctx_buffers(const ace_ctx* ctx) {
+    Workspace w;
+    w.Aplus = ctx->Aplus.p;
+    w.Gbuf = ctx->Gbuf.p;
+    w.Epart = ctx->Epart.p;
+    w.nbr_env = ctx->nbr_env.p;
+    w.nbr_cap = ctx->nbr_env.n;
+    return w;
+}
+
 inline int set_func_lds(const void* f, size_t bytes) {
     if (bytes > 64 * 1024) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return 0;
@@ -333,13 +357,13 @@ int validate_inputs(ace_ctx* ctx, int64_t nenv, const int32_t* off, int64_t nnbr
 // (device count env_count): energies, and the adjoint G with forces.
 int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estride, bool forces, double* E_out,
                       const int32_t* env_list, const int32_t* env_count);
-// ace_pool.hip: A+[k][env] into ctx->Aplus (stride estride); with nbr_map
-// also the neighbour -> environment map ctx->nbr_env (flat force kernel).
+// ace_pool.hip: A+[k][env] into ws.Aplus (stride estride); with nbr_map
+// also the neighbour -> environment map ws.nbr_env (flat force kernel).
 int launch_pool_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
-                    int64_t nnbr, const int8_t* species, int64_t estride, bool nbr_map);
+                    int64_t nnbr, const int8_t* species, int64_t estride, const Workspace& ws, bool nbr_map);
 // ace_pool.hip: dA+w[k][env] = sum_j w_j . grad_j A+[k] into ctx->Wplus.
 int launch_poolw_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
                      const double* w, int64_t nnbr, const int8_t* species, int64_t estride);
-// ace_force.hip: +dE/dr_j from the adjoint ctx->Gbuf.
+// ace_force.hip: +dE/dr_j from the adjoint ws.Gbuf (and ws.nbr_env).
 int launch_force_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
-                     int64_t nnbr, const int8_t* species, int64_t estride, double* dE);
+                     int64_t nnbr, const int8_t* species, int64_t estride, const Workspace& ws, double* dE);

@@ -517,32 +517,32 @@ __global__ void k_gfold(int64_t nenv, int nfold, const int32_t* __restrict__ fol
 template <int L, int PROP>
 static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
                          const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride,
-                         double* dE) {
+                         const Workspace& ws, double* dE) {
     constexpr int NC = PROP == 0 ? 1 : 3;
     constexpr int NP = (L + 1) * (L + 2) / 2;
     if (PROP == 0 && ctx->flat_force) {
         if (ctx->nfold > 0)
             hipLaunchKernelGGL(k_gfold, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, ctx->nfold,
-                               ctx->gfold.p, ctx->Gbuf.p, estride);
+                               ctx->gfold.p, ws.Gbuf, estride);
         // radial functions held in registers: 12 when they suffice (fewer
         // registers, more waves per SIMD), else 16
         // blocks of short environments span more of them: stage 32 when that fits
         const bool wide = (size_t)ctx->fflat.nq * FF_MAXE_WIDE * sizeof(double2) <= FF_LDS;
         const size_t glds = (size_t)ctx->fflat.nq * (wide ? FF_MAXE_WIDE : FF_MAXE) * sizeof(double2);
         const dim3 grid((unsigned)((nnbr + 255) / 256));
-        const int32_t* ne_ = ctx->nbr_env.p;
+        const int32_t* ne_ = ws.nbr_env;
         if (ctx->nr <= 12 && wide)
             hipLaunchKernelGGL((k_forcef<L, 12, FF_MAXE_WIDE>), grid, dim3(256), glds, st, radial_params(ctx),
-                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
         else if (ctx->nr <= 12)
             hipLaunchKernelGGL((k_forcef<L, 12, FF_MAXE>), grid, dim3(256), glds, st, radial_params(ctx),
-                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
         else if (wide)
             hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX, FF_MAXE_WIDE>), grid, dim3(256), glds, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
         else
             hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX, FF_MAXE>), grid, dim3(256), glds, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ctx->Gbuf.p, estride, dE);
+                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
         return;
     }
     const int nsp = std::max(1, ctx->nspecies);
@@ -553,25 +553,25 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
         set_func_lds((const void*)k_force<L, PROP>, dense);
         hipLaunchKernelGGL((k_force<L, PROP>), dim3((unsigned)nenv), dim3(64), dense, st, radial_params(ctx), ft,
                            nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, nsp, ctx->pool.p,
-                           ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ctx->Gbuf.p, estride, dE);
+                           ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ws.Gbuf, estride, dE);
         return;
     }
     const size_t lds = (size_t)ctx->npool * NC * sizeof(double2) + 64 * 2 * ctx->nr * sizeof(double);
     set_func_lds((const void*)k_force_lds<L, PROP>, lds);
     hipLaunchKernelGGL((k_force_lds<L, PROP>), dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx),
                        nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->kidx.p,
-                       ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ctx->Gbuf.p, estride, dE);
+                       ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ws.Gbuf, estride, dE);
 }
 
 }  // namespace
 
 template <int L>
 int launch_force_l(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
-                   int64_t nnbr, const int8_t* species, int64_t estride, double* dE) {
+                   int64_t nnbr, const int8_t* species, int64_t estride, const Workspace& ws, double* dE) {
     if (ctx->prop == ACE_PROP_INVARIANT)
-        launch_force<L, 0>(ctx, st, nenv, off, xyz, nnbr, species, estride, dE);
+        launch_force<L, 0>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
     else
-        launch_force<L, 1>(ctx, st, nenv, off, xyz, nnbr, species, estride, dE);
+        launch_force<L, 1>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
     HIP_TRY(hipGetLastError());
     return 0;
 }

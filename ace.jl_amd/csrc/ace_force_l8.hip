@@ -2,4 +2,4 @@
 #include "ace_force_impl.hpp"
 
 template int launch_force_l<8>(ace_ctx*, hipStream_t, int64_t, const int32_t*, const double*, int64_t,
-                                 const int8_t*, int64_t, double*);
+                                 const int8_t*, int64_t, const Workspace&, double*);

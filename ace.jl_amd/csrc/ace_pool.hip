@@ -267,32 +267,34 @@ __global__ __launch_bounds__(64) void k_poolw(RadialParams rp, int64_t nenv, con
 
 template <int L>
 static int launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
-                       const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride, bool nbr_map) {
+                       const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride, const Workspace& ws,
+                       bool nbr_map) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
     // two environments per wave when a half-wave covers the pooled functions
     const int epw = ctx->npool <= 32 ? 2 : 1;
     const size_t lds = (size_t)epw * POOL_CH * (ctx->nr + 2 * NP + 1) * sizeof(double);
     int32_t* nbr_env = nullptr;
     if (nbr_map && nnbr > 0) {
-        if (ctx->nbr_env.ensure(nnbr)) return -1;
-        nbr_env = ctx->nbr_env.p;
+        if (!ws.nbr_env || ws.nbr_cap < (size_t)nnbr) return set_err("neighbour map buffer too small");
+        nbr_env = ws.nbr_env;
     }
+    if (!ws.Aplus) return set_err("no buffer for the pooled values");
     const dim3 grid((unsigned)((nenv + epw - 1) / epw));
     const int8_t* sp = ctx->nspecies > 0 ? species : nullptr;
     // pooled functions per lane: 1 (two environments per wave), 4 or 8
     if (epw == 2) {
         set_func_lds((const void*)k_pool<L, 2, 1>, lds);
         hipLaunchKernelGGL((k_pool<L, 2, 1>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
-                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+                           ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
     } else if (ctx->npool <= 256) {
         set_func_lds((const void*)k_pool<L, 1, 4>, lds);
         hipLaunchKernelGGL((k_pool<L, 1, 4>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
-                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+                           ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
     } else {
         if (ctx->npool > 512) return set_err("pooled 1p basis too large (npool > 512)");
         set_func_lds((const void*)k_pool<L, 1, 8>, lds);
         hipLaunchKernelGGL((k_pool<L, 1, 8>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
-                           ctx->pool.p, ctx->npool, ctx->Aplus.p, estride, nbr_env);
+                           ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -319,17 +321,17 @@ static int launch_poolw(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_
 }
 
 int launch_pool_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
-                    int64_t nnbr, const int8_t* species, int64_t estride, bool nbr_map) {
+                    int64_t nnbr, const int8_t* species, int64_t estride, const Workspace& ws, bool nbr_map) {
     switch (ctx->L) {
-        case 0: return launch_pool<0>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        case 1: return launch_pool<1>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        case 2: return launch_pool<2>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        case 3: return launch_pool<3>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        case 4: return launch_pool<4>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        case 5: return launch_pool<5>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        case 6: return launch_pool<6>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        case 7: return launch_pool<7>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
-        default: return launch_pool<8>(ctx, st, nenv, off, xyz, nnbr, species, estride, nbr_map);
+        case 0: return launch_pool<0>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 1: return launch_pool<1>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 2: return launch_pool<2>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 3: return launch_pool<3>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 4: return launch_pool<4>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 5: return launch_pool<5>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 6: return launch_pool<6>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 7: return launch_pool<7>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        default: return launch_pool<8>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
     }
 }
 

@@ -169,7 +169,9 @@ extern "C" int ace_surr_features(ace_ctx* ctx, int64_t nenv, const int32_t* off,
     const int64_t estride = ((nenv + 63) / 64) * 64;
     if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
     // the neighbour map is written for the force kernel of the following call
-    if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx->flat_force)) return -1;
+    if (ctx->flat_force && nnbr > 0 && ctx->nbr_env.ensure(nnbr)) return -1;
+    if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), ctx->flat_force))
+        return -1;
     hipLaunchKernelGGL(k_feat, dim3((unsigned)((nenv + 63) / 64)), dim3(64), 0, st, feat_args(ctx), nenv,
                        ctx->Aplus.p, estride, xmean, xinvstd, X, (__bf16*)Xn, E2, ctx->e0v.p);
     HIP_TRY(hipGetLastError());
@@ -209,6 +211,7 @@ extern "C" int ace_surr_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t*
     }
     // plain ACE for the listed environments (energy, and G for forces)
     if (launch_dag_listed(ctx, st, nenv, estride, forces, E_out, fb_list, fb_count)) return -1;
-    if (forces && nnbr > 0 && launch_force_any(ctx, st, nenv, off, xyz, nnbr, species, estride, dE)) return -1;
+    if (forces && nnbr > 0 && launch_force_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), dE))
+        return -1;
     return 0;
 }

@@ -747,7 +747,7 @@ int validate_inputs(ace_ctx* ctx, int64_t nenv, const int32_t* off, int64_t nnbr
 // prefix products up to depth 3 or 7.
 template <int PROP, int ENVS, int MAXD>
 static int launch_dag_tile(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int64_t nenv, int64_t estride,
-                           double* E_out) {
+                           const Workspace& ws, double* E_out) {
     constexpr int NC = PROP == 0 ? 1 : 3;
     constexpr int SPW = 64 / ENVS;  // task streams per wave
     if (a.W % SPW) return set_err("tile configuration needs a multiple of 64 / ENVS task streams");
@@ -755,56 +755,51 @@ static int launch_dag_tile(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int64
     const int64_t nt = (nenv + ENVS - 1) / ENVS;
     set_func_lds((const void*)k_dag<PROP, ENVS, MAXD>, lds);
     hipLaunchKernelGGL((k_dag<PROP, ENVS, MAXD>), dim3((unsigned)nt), dim3(64 * (a.W / SPW)), lds, st, a, nenv,
-                       ctx->Aplus.p, estride, E_out, ctx->Gbuf.p, ctx->e0v.p);
+                       ws.Aplus, estride, E_out, ws.Gbuf, ctx->e0v.p);
     return 0;
 }
 
 template <int PROP, int MAXD>
 static int launch_dag_prop(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int64_t nenv, int64_t estride,
-                           double* E_out) {
+                           const Workspace& ws, double* E_out) {
     constexpr int NC = PROP == 0 ? 1 : 3;
     auto fits = [&](int envs) {
         return std::max((size_t)ctx->npool * envs * sizeof(double2), (size_t)a.W * NC * envs * sizeof(double)) <=
                160 * 1024;
     };
     ctx->last_kernel = fits(64) ? 64 : fits(32) ? 32 : 16;
-    if (fits(64)) return launch_dag_tile<PROP, 64, MAXD>(ctx, st, a, nenv, estride, E_out);
-    if (fits(32)) return launch_dag_tile<PROP, 32, MAXD>(ctx, st, a, nenv, estride, E_out);
-    if (fits(16)) return launch_dag_tile<PROP, 16, MAXD>(ctx, st, a, nenv, estride, E_out);
+    if (fits(64)) return launch_dag_tile<PROP, 64, MAXD>(ctx, st, a, nenv, estride, ws, E_out);
+    if (fits(32)) return launch_dag_tile<PROP, 32, MAXD>(ctx, st, a, nenv, estride, ws, E_out);
+    if (fits(16)) return launch_dag_tile<PROP, 16, MAXD>(ctx, st, a, nenv, estride, ws, E_out);
     return set_err("pooled 1p basis too large for the LDS tile of 16 environments");
 }
 
 static int launch_dag_interp(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int64_t nenv, int64_t estride,
-                             double* E_out) {
+                             const Workspace& ws, double* E_out) {
     const bool deep = ctx->maxorder > 4;
     if (ctx->prop == ACE_PROP_INVARIANT)
-        return deep ? launch_dag_prop<0, 7>(ctx, st, a, nenv, estride, E_out)
-                    : launch_dag_prop<0, 3>(ctx, st, a, nenv, estride, E_out);
-    return deep ? launch_dag_prop<1, 7>(ctx, st, a, nenv, estride, E_out)
-                : launch_dag_prop<1, 3>(ctx, st, a, nenv, estride, E_out);
+        return deep ? launch_dag_prop<0, 7>(ctx, st, a, nenv, estride, ws, E_out)
+                    : launch_dag_prop<0, 3>(ctx, st, a, nenv, estride, ws, E_out);
+    return deep ? launch_dag_prop<1, 7>(ctx, st, a, nenv, estride, ws, E_out)
+                : launch_dag_prop<1, 3>(ctx, st, a, nenv, estride, ws, E_out);
 }
 
-static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
-                    const int8_t* species, double* E_out, double* dE, void* stream) {
-    int64_t nnbr = 0;
-    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
-    if (validate_inputs(ctx, nenv, off, nnbr, species, (hipStream_t)stream)) return -1;
-    if (nenv == 0) return 0;
-    if (!E_out) return set_err("null E_out");
-    hipStream_t st = (hipStream_t)stream;
+// The launches of an energy (+ forces) evaluation, into a workspace that
+// holds A+ [npool][estride], G [ngslots * NC][estride] (forces), MAX_PARTS
+// energy shares [estride] and, for the flat force kernel, nnbr map entries.
+static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
+                     int64_t nnbr, const int8_t* species, double* E_out, double* dE, const Workspace& ws) {
     const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
     const int64_t ntile = (nenv + 63) / 64;
     const int64_t estride = ntile * 64;
-    if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
     const bool forces = dE != nullptr;
-    if (forces && ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * NC * estride)) return -1;
     WaveCfg* wc = ctx->wcfg[ctx->cur_w];
     // under stream capture (hipGraph) the timing events are left out of the graph
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     HIP_TRY(hipStreamIsCapturing(st, &cap));
     const bool timed = cap == hipStreamCaptureStatusNone;
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[0], st));
-    if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, forces && ctx->flat_force)) return -1;
+    if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, forces && ctx->flat_force)) return -1;
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[1], st));
     DagArgs a;
     a.runs = ctx->runs.p;
@@ -875,26 +870,41 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
         ra.ntile_lo = ntile - nhi;
         ra.wtask_ptr_hi = nhi > 0 ? (forces ? wc->pp_all[P + 1].p : wc->pp_e[P + 1].p) : nullptr;
         ra.wtask_list_hi = nhi > 0 ? (forces ? wc->pl_all[P + 1].p : wc->pl_e[P + 1].p) : nullptr;
-        const int nshare = nhi > 0 ? P + 1 : P;
-        if (P > 1 && ctx->Epart.ensure((size_t)nshare * estride)) return -1;
         const int64_t nblk = (ntile - nhi) * P + nhi * (P + 1);
         set_func_lds((const void*)k_dagr, lds);
         hipLaunchKernelGGL(k_dagr, dim3((unsigned)nblk), dim3(64 * wc->W), lds, st, ra, nenv,
-                           ctx->Aplus.p, estride, P == 1 ? E_out : ctx->Epart.p, ctx->Gbuf.p, ctx->e0v.p,
+                           ws.Aplus, estride, P == 1 ? E_out : ws.Epart, ws.Gbuf, ctx->e0v.p,
                            (const int32_t*)nullptr, (const int32_t*)nullptr);
         if (P > 1)
             hipLaunchKernelGGL(k_esum, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, P,
-                               (ntile - nhi) * 64, ctx->Epart.p, estride, ctx->e0v.p, E_out);
+                               (ntile - nhi) * 64, ws.Epart, estride, ctx->e0v.p, E_out);
     } else {
         ctx->last_parts = 1;
-        if (launch_dag_interp(ctx, st, a, nenv, estride, E_out)) return -1;
+        if (launch_dag_interp(ctx, st, a, nenv, estride, ws, E_out)) return -1;
     }
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[2], st));
-    if (forces && nnbr > 0 && launch_force_any(ctx, st, nenv, off, xyz, nnbr, species, estride, dE)) return -1;
+    if (forces && nnbr > 0 && launch_force_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE)) return -1;
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[3], st));
     ctx->timing_valid = timed;
     return 0;
+}
+
+static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
+                    const int8_t* species, double* E_out, double* dE, void* stream) {
+    int64_t nnbr = 0;
+    if (check_common(ctx, nenv, off, xyz, species, &nnbr, stream)) return -1;
+    if (validate_inputs(ctx, nenv, off, nnbr, species, (hipStream_t)stream)) return -1;
+    if (nenv == 0) return 0;
+    if (!E_out) return set_err("null E_out");
+    const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
+    const int64_t estride = ((nenv + 63) / 64) * 64;
+    const bool forces = dE != nullptr;
+    if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
+    if (forces && ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * NC * estride)) return -1;
+    if (ctx->prop == ACE_PROP_INVARIANT && ctx->Epart.ensure((size_t)MAX_PARTS * estride)) return -1;
+    if (forces && ctx->flat_force && nnbr > 0 && ctx->nbr_env.ensure(nnbr)) return -1;
+    return eval_path(ctx, (hipStream_t)stream, nenv, off, xyz, nnbr, species, E_out, dE, ctx_buffers(ctx));
 }
 
 extern "C" int ace_energy(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
@@ -946,7 +956,7 @@ int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estrid
     if (lds > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     set_func_lds((const void*)k_dagr, lds);
     hipLaunchKernelGGL(k_dagr, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, ra, nenv, ctx->Aplus.p, estride,
-                       E_out, ctx->Gbuf.p, ctx->e0v.p, env_list, env_count);
+                       E_out, forces ? ctx->Gbuf.p : nullptr, ctx->e0v.p, env_list, env_count);
     HIP_TRY(hipGetLastError());
     return 0;
 }
