@@ -68,7 +68,7 @@ EXPORTS = ("ace_ctx_create", "ace_ctx_create_jl", "ace_compile_check", "ace_last
            "ace_last_timing", "ace_set_validation", "ace_last_error",
            "ace_ctx_destroy", "ace_abi_version", "ace_nbrlist_create", "ace_nbrlist_destroy",
            "ace_nbrlist_build", "ace_nbrlist_update", "ace_nbrlist_forces", "ace_nbrlist_forces_multi",
-           "ace_adjoint_EVAL_D")
+           "ace_adjoint_EVAL_D", "ace_energy_forces_noalloc")
 
 ERR_CAPACITY = -2  # ACE_ERR_CAPACITY
 
@@ -100,6 +100,7 @@ def load(path: str | None = None):
     ev = [vp, C.c_int64, vp, vp, vp, vp, vp]
     lib.ace_energy.argtypes = ev
     lib.ace_energy_forces.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
+    lib.ace_energy_forces_noalloc.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, vp, vp, vp]
     for name in ("ace_basis_A", "ace_basis_AA", "ace_basis_B", "ace_basis_dB"):
         getattr(lib, name).argtypes = ev
     lib.ace_surr_features.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp]

@@ -164,6 +164,14 @@ class HIPEvaluator:
                                               _vp(b.species), _vp(E), _vp(dE), self._stream(stream)))
         return E, dE
 
+    def energy_forces_noalloc(self, b: EnvBatch, E, dE, stream=None):
+        """``ace_energy_forces_noalloc``: the same results as energy_forces,
+        with the library allocating nothing (its scratch is a fixed device
+        array; too large a batch raises).  E and dE are the caller's."""
+        _lib.check(self.lib.ace_energy_forces_noalloc(self.ctx, b.nenv, _vp(b.offsets), b.nnbr, _vp(b.xyz),
+                                                      _vp(b.species), _vp(E), _vp(dE), self._stream(stream)))
+        return E, dE
+
     def last_timing(self):
         a, b_, c = C.c_double(), C.c_double(), C.c_double()
         _lib.check(self.lib.ace_last_timing(self.ctx, C.byref(a), C.byref(b_), C.byref(c)))

@@ -263,6 +263,7 @@ struct ace_ctx {
     int64_t last_tail = 0;  // tiles of the last call that ran one more part
     int last_kernel = 0;    // product-DAG kernel of the last call: 0 records, else the interpreter's tile width
     int nrec = 0;           // program records (0: the run interpreter evaluates every call)
+    unsigned char* scratch = nullptr;  // device address of this device's allocation-free scratch array
     // scratch
     DevBuf<double> Epart;
     DevBuf<double2> Aplus, Gbuf, Afull, AA, dphi, dAA, Wplus, Wfull;

@@ -329,6 +329,13 @@ __global__ void k_esum(int64_t nenv, int parts, int64_t env_hi, const double* __
 }
 
 // ---------------------------------------------------------------------------
+// The scratch of ace_energy_forces_noalloc: a fixed device array, one copy
+// per device, held for the life of the process.  Host code reaches it only
+// through the address hipGetSymbolAddress returns (ace_ctx::scratch).
+constexpr size_t ACE_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[ACE_SCRATCH_BYTES];
+
+// ---------------------------------------------------------------------------
 // host side
 
 static int upload_coeffs(ace_ctx* ctx, const double* c) {
@@ -431,6 +438,11 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         delete ctx;
         return set_err(std::string("ace_ctx_create: ") + what + ": " + msg);
     };
+    // this device's copy of the allocation-free entry point's scratch array
+    if (hipGetSymbolAddress((void**)&ctx->scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
+        delete ctx;
+        return set_err("ace_ctx_create: hipGetSymbolAddress of the scratch array failed");
+    }
     const std::vector<double> alp = alp_table();
     if (hipHostMalloc((void**)&ctx->h_words, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
         ctx->h_words = nullptr;
@@ -905,6 +917,38 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     if (ctx->prop == ACE_PROP_INVARIANT && ctx->Epart.ensure((size_t)MAX_PARTS * estride)) return -1;
     if (forces && ctx->flat_force && nnbr > 0 && ctx->nbr_env.ensure(nnbr)) return -1;
     return eval_path(ctx, (hipStream_t)stream, nenv, off, xyz, nnbr, species, E_out, dE, ctx_buffers(ctx));
+}
+
+extern "C" int ace_energy_forces_noalloc(ace_ctx* ctx, int64_t nenv, const int32_t* off, int64_t nnbr,
+                                         const double* xyz, const int8_t* species, double* E_out, double* dE_dr,
+                                         void* stream) {
+    if (!ctx) return set_err("ace_energy_forces_noalloc: null context");
+    if (nenv < 0 || nnbr < 0) return set_err("ace_energy_forces_noalloc: negative size");
+    if (nenv == 0) return 0;
+    if (!off || !E_out || !dE_dr || (nnbr > 0 && !xyz)) return set_err("ace_energy_forces_noalloc: null argument");
+    if (ctx->nspecies > 0 && nnbr > 0 && !species)
+        return set_err("species array required for a categorical 1p basis");
+    if (!ctx->scratch) return set_err("ace_energy_forces_noalloc: no scratch array on this device");
+    HIP_TRY(hipSetDevice(ctx->device));
+    // carve the fixed array: A+, G, the energy shares, the neighbour map
+    const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
+    const int64_t estride = ((nenv + 63) / 64) * 64;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bA = up((size_t)ctx->npool * estride * sizeof(double2));
+    const size_t bG = up((size_t)std::max(1, ctx->ngslots) * NC * estride * sizeof(double2));
+    const size_t bE = ctx->prop == ACE_PROP_INVARIANT ? up((size_t)MAX_PARTS * estride * sizeof(double)) : 0;
+    const size_t bN = ctx->flat_force ? up((size_t)nnbr * sizeof(int32_t)) : 0;
+    const size_t need = bA + bG + bE + bN;
+    if (need > ACE_SCRATCH_BYTES)
+        return set_err("ace_energy_forces_noalloc: the batch needs " + std::to_string(need) + " bytes of scratch, " +
+                       "more than the fixed " + std::to_string(ACE_SCRATCH_BYTES) + "; split it into smaller batches");
+    Workspace ws;
+    ws.Aplus = reinterpret_cast<double2*>(ctx->scratch);
+    ws.Gbuf = reinterpret_cast<double2*>(ctx->scratch + bA);
+    ws.Epart = bE ? reinterpret_cast<double*>(ctx->scratch + bA + bG) : nullptr;
+    ws.nbr_env = bN ? reinterpret_cast<int32_t*>(ctx->scratch + bA + bG + bE) : nullptr;
+    ws.nbr_cap = bN ? (size_t)nnbr : 0;
+    return eval_path(ctx, (hipStream_t)stream, nenv, off, xyz, nnbr, species, E_out, dE_dr, ws);
 }
 
 extern "C" int ace_energy(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,

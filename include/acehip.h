@@ -243,6 +243,21 @@ int ace_energy_forces(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
                       const double* xyz, const int8_t* species, double* E_out,
                       double* dE_dr, void* stream);
 
+/* Energies and gradients as ace_energy_forces, allocating nothing: no
+ * device or host-pinned allocation and no host read of the inputs, so the
+ * call can run under HIP graph capture or in a loop that must not allocate.
+ * No counterpart in the reference; for callers that cannot allocate.  The
+ * caller passes nnbr = env_offsets[nenv] (the offsets are not validated).
+ * The scratch (pooled values, adjoint, energy shares, neighbour map) lives
+ * in a fixed 64 MiB device array, one per device for the life of the
+ * process: a batch that needs more is rejected with an error (at the
+ * order-4 degree-12 basis about 12,000 environments of 30 neighbours fit).
+ * Every call on one device shares that array, so calls that overlap on one
+ * device (other contexts or streams included) race. */
+int ace_energy_forces_noalloc(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, int64_t nnbr,
+                              const double* xyz, const int8_t* species, double* E_out, double* dE_dr,
+                              void* stream);
+
 /* Product1pBasis evaluate (product_1pbasis.jl:123-134): A[nenv][nA] complex. */
 int ace_basis_A(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
                 const double* xyz, const int8_t* species, double* A_out,
