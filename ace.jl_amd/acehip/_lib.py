@@ -68,7 +68,8 @@ EXPORTS = ("ace_ctx_create", "ace_ctx_create_jl", "ace_compile_check", "ace_last
            "ace_last_timing", "ace_set_validation", "ace_last_error",
            "ace_ctx_destroy", "ace_abi_version", "ace_nbrlist_create", "ace_nbrlist_destroy",
            "ace_nbrlist_build", "ace_nbrlist_update", "ace_nbrlist_forces", "ace_nbrlist_forces_multi",
-           "ace_adjoint_EVAL_D", "ace_energy_forces_noalloc")
+           "ace_adjoint_EVAL_D", "ace_energy_forces_noalloc", "ace_energy_host", "ace_energy_forces_host",
+           "ace_basis_B_host", "ace_basis_dB_host", "ace_adjoint_EVAL_D_host")
 
 ERR_CAPACITY = -2  # ACE_ERR_CAPACITY
 
@@ -122,6 +123,11 @@ def load(path: str | None = None):
     lib.ace_nbrlist_forces.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp]
     lib.ace_nbrlist_forces_multi.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, C.c_int64, vp, vp, vp]
     lib.ace_adjoint_EVAL_D.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
+    lib.ace_energy_host.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
+    lib.ace_energy_forces_host.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp]
+    lib.ace_basis_B_host.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
+    lib.ace_basis_dB_host.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
+    lib.ace_adjoint_EVAL_D_host.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp]
     if lib.ace_abi_version() != ABI_VERSION:
         raise AceError("libacehip.so ABI version mismatch")
     if path is None:

@@ -268,6 +268,11 @@ struct ace_ctx {
     DevBuf<double> Epart;
     DevBuf<double2> Aplus, Gbuf, Afull, AA, dphi, dAA, Wplus, Wfull;
     DevBuf<int32_t> nbr_env;
+    // host-memory calls (ace_*_host): their stream and device staging buffers
+    hipStream_t io_stream = nullptr;
+    DevBuf<int32_t> io_off;
+    DevBuf<double> io_xyz, io_w, io_out1, io_out2;
+    DevBuf<int8_t> io_sp;
     // pinned host words for the stream-ordered reads of env_offsets[0] and
     // env_offsets[nenv] (hipHostMalloc)
     int32_t* h_words = nullptr;
@@ -277,6 +282,7 @@ struct ace_ctx {
     bool validate = true;
     ~ace_ctx() {
         if (h_words) (void)hipHostFree(h_words);
+        if (io_stream) (void)hipStreamDestroy(io_stream);
         for (auto* w : wcfg) delete w;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);

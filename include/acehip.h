@@ -258,6 +258,23 @@ int ace_energy_forces_noalloc(ace_ctx* ctx, int64_t nenv, const int32_t* env_off
                               const double* xyz, const int8_t* species, double* E_out, double* dE_dr,
                               void* stream);
 
+/* Host-memory variants for callers without device buffers (a Julia ccall
+ * with Base arrays): every array argument is a HOST pointer with the layout
+ * of the device entry point of the same name; env_offsets and species are
+ * checked on the host, the inputs are staged to the context's device
+ * buffers on the context's own stream, and the call returns after the
+ * outputs are back (PCIe-bound: about 50 bytes per neighbour). */
+int ace_energy_host(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                    const int8_t* species, double* E_out);
+int ace_energy_forces_host(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                           const int8_t* species, double* E_out, double* dE_dr);
+int ace_basis_B_host(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                     const int8_t* species, double* B_out);
+int ace_basis_dB_host(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                      const int8_t* species, double* dB_out);
+int ace_adjoint_EVAL_D_host(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets, const double* xyz,
+                            const int8_t* species, const double* w, double* dB_out);
+
 /* Product1pBasis evaluate (product_1pbasis.jl:123-134): A[nenv][nA] complex. */
 int ace_basis_A(ace_ctx* ctx, int64_t nenv, const int32_t* env_offsets,
                 const double* xyz, const int8_t* species, double* A_out,

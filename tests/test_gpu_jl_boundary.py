@@ -135,3 +135,56 @@ def test_two_species_order4_degree12_vs_oracle(cuda):
             g_ref.append(O.grad_config(T, c, oe, Re, se))
     assert rel_err(E[picks], np.asarray(E_ref)) < E_TOL
     assert rel_err(np.concatenate(g), np.concatenate(g_ref)) < F_TOL
+
+
+@pytest.mark.parametrize("args,kw", [((3, 6, 4), {}), ((3, 6, 3), {"species": 2}),
+                                     ((3, 6, 4), {"prop": "EuclideanVector"})])
+def test_host_memory_entry_points(cuda, args, kw):
+    """The host-memory calls a Julia caller uses (ace_*_host: host arrays in and
+    out) return exactly what the device entry points return."""
+    import ctypes as C
+    print(f"[gpu] host-memory entry points {args} {kw}", flush=True)
+    basis, T = basis_for(*args, **kw)
+    c = synth.coefficients(len(basis), 21)
+    ns = kw.get("species", 0)
+    off, R, sp = random_batch(70, 22, nspecies=ns)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    nc = 1 if ev.prop == 0 else 3
+    nnbr = int(off[-1])
+    E_d, dE_d = ev.energy_forces(b)
+    B_d = ev.basis_B(b)
+    dB_d = ev.basis_dB(b)
+    torch.cuda.synchronize()
+    lib = ev.lib
+    p = lambda a: C.c_void_p(a.ctypes.data) if a is not None else C.c_void_p(0)  # noqa: E731
+    off32 = np.ascontiguousarray(off, dtype=np.int32)
+    xyz = np.ascontiguousarray(R.T, dtype=np.float64)
+    spa = None if sp is None else np.ascontiguousarray(sp, dtype=np.int8)
+    E = np.zeros(70 * nc)
+    g = np.zeros(3 * nc * nnbr)
+    from acehip._lib import check
+    check(lib.ace_energy_forces_host(ev.ctx, 70, p(off32), p(xyz), p(spa), p(E), p(g)))
+    assert np.array_equal(E, E_d.cpu().numpy().reshape(-1))
+    assert np.array_equal(g, dE_d.cpu().numpy().reshape(-1))
+    E1 = np.zeros(70 * nc)
+    check(lib.ace_energy_host(ev.ctx, 70, p(off32), p(xyz), p(spa), p(E1)))
+    assert rel_err(E1, E) < E_TOL
+    B = np.zeros(70 * ev.nB * nc)
+    check(lib.ace_basis_B_host(ev.ctx, 70, p(off32), p(xyz), p(spa), p(B)))
+    assert np.array_equal(B, B_d.cpu().numpy().reshape(-1))
+    dB = np.zeros(nnbr * ev.nB * 3 * nc)
+    check(lib.ace_basis_dB_host(ev.ctx, 70, p(off32), p(xyz), p(spa), p(dB)))
+    assert np.array_equal(dB, dB_d.cpu().numpy().reshape(-1))
+    if ev.prop == 0:
+        w = np.random.default_rng(3).normal(size=(3, nnbr))
+        out = np.zeros(70 * ev.nB)
+        check(lib.ace_adjoint_EVAL_D_host(ev.ctx, 70, p(off32), p(xyz), p(spa), p(np.ascontiguousarray(w)), p(out)))
+        ref = ev.adjoint_EVAL_D(b, torch.tensor(w)).cpu().numpy().reshape(-1)
+        torch.cuda.synchronize()
+        assert np.array_equal(out, ref)
+    bad = off32.copy()
+    bad[3] = bad[2] - 1
+    from acehip._lib import AceError
+    with pytest.raises(AceError, match="non-decreasing"):
+        check(lib.ace_energy_host(ev.ctx, 70, p(bad), p(xyz), p(spa), p(E1)))
