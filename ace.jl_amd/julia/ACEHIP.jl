@@ -168,7 +168,8 @@ function HIPEvaluator(basis::SymmetricBasis, c::AbstractVector; device::Integer 
     cv = Vector{Float64}(c)
     length(cv) == info.nB || error("ACEHIP: length(c) != length(basis)")
     out = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve arrays cv begin
+    a1, a2, a3, a4, a5, a6, a7, a8, a9, a10 = arrays
+    GC.@preserve a1 a2 a3 a4 a5 a6 a7 a8 a9 a10 cv begin
         jb = Ref(_jl_basis(arrays, info))
         _check(ccall((:ace_ctx_create_jl, libacehip), Cint,
                      (Ref{AceJlBasis}, Cint, Ptr{Float64}, Int64, Ref{Ptr{Cvoid}}),
