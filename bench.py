@@ -30,6 +30,15 @@ FP64_PEAK_TFLOPS = 78.6
 # 1.0 for WRITE_SIZE): 714.9 MB read + 269.7 MB written.  Only valid for the
 # default --nenv; re-measured whenever the kernel changes.
 DAG_TRAFFIC_BYTES = {100_000: 984.5e6}
+WORKLOAD_NAMES = {
+    1: "plain ACE energy+forces, 1000 envs x 12 neighbours, order 3, degree 6, Lmax 4, Invariant, FP64",
+    2: "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], order 4, degree 12, Lmax 6, "
+       "Invariant, FP64",
+    3: "plain ACE energy+forces, 32,768 FCC environments (a 3.6 A, 0.1 A jitter, rcut 5 A), order 3, degree 6, "
+       "Lmax 4, Invariant, FP64",
+    4: "plain ACE energy+forces, 65,536 envs, 2 species, nX~Poisson(40) clipped [5,80], order 3, degree 10, "
+       "Lmax 5, EuclideanVector, FP64",
+}
 TRAFFIC_SOURCE = "profiles/plain_ef_config2_kernels.md"
 
 
@@ -57,7 +66,21 @@ def executed_flops(pack, forces=True):
     return 4.0 * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
 
 
-def cpu_baseline(T, c, off, R, budget_s=12.0):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(T, c, off, R, sp=None, budget_s=15.0, runs=5):
+    """The C++/OpenMP ProductEvaluator port on the host cores: one warm-up
+    run, then the median of ``runs`` timed runs over a sample of the batch
+    sized so the runs take about ``budget_s`` together."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from ace_cpu import CpuModel
     import numpy as np
@@ -66,19 +89,23 @@ def cpu_baseline(T, c, off, R, budget_s=12.0):
     threads = min(int(os.environ.get("OMP_NUM_THREADS", ncpu)), ncpu)
     # probe the rate on a small sample, then size the timed sample to the budget
     n0 = min(256, len(off) - 1)
-    m.eval(off[: n0 + 1], R[: off[n0]], forces=True, nthreads=threads)
+    spn = lambda n: None if sp is None else sp[: off[n]]  # noqa: E731
+    m.eval(off[: n0 + 1], R[: off[n0]], forces=True, nthreads=threads, species=spn(n0))
     t = time.perf_counter()
-    m.eval(off[: n0 + 1], R[: off[n0]], forces=True, nthreads=threads)
+    m.eval(off[: n0 + 1], R[: off[n0]], forces=True, nthreads=threads, species=spn(n0))
     rate = n0 / (time.perf_counter() - t)
-    n = int(min(len(off) - 1, max(n0, rate * budget_s)))
-    reps = max(1, int(round(rate * budget_s / n)))
-    t = time.perf_counter()
-    for _ in range(reps):
-        m.eval(off[: n + 1], R[: off[n]], forces=True, nthreads=threads)
-    dt = time.perf_counter() - t
-    return {"value": n * reps / dt, "unit": "env/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} environments of the same batch x {reps}, energy+forces, "
-                      f"C++/OpenMP ProductEvaluator port, {dt:.1f} s"}
+    n = int(min(len(off) - 1, max(n0, rate * budget_s / runs)))
+    m.eval(off[: n + 1], R[: off[n]], forces=True, nthreads=threads, species=spn(n))  # warm-up
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        m.eval(off[: n + 1], R[: off[n]], forces=True, nthreads=threads, species=spn(n))
+        times.append(time.perf_counter() - t)
+    med = float(np.median(times))
+    return {"value": n / med, "unit": "env/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "runs": runs, "run_s": [round(x, 3) for x in times],
+            "sample": f"first {n} environments of the same batch, energy+forces, median of {runs} runs "
+                      f"after one warm-up, C++/OpenMP ProductEvaluator port, {threads} threads"}
 
 
 def surrogate_leg(ev, b, E_plain, steps, warmup, ntrain, neval, epochs, log):
@@ -193,7 +220,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nenv", type=int, default=100_000)
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4),
+                    help="workload: 2 (default) is the one the metric is quoted on; 1, 3 and 4 are the "
+                         "other configurations, timed with the same step for their profiles")
+    ap.add_argument("--nenv", type=int, default=0, help="environments (0: the workload's own size)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--waves", type=int, default=0, help="tile waves (0 = autotune)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
@@ -217,14 +247,16 @@ def main():
     from acehip.tables import basis_tables
 
     dev = torch.device("cuda:0")
-    w = synth.WORKLOADS[2]
+    w = synth.WORKLOADS[args.config]
+    headline = args.config == 2
     basis = synth.make_basis(w)
     c = synth.coefficients(len(basis), w["seed"])
-    off, R, sp = synth.workload_envs(2, nenv=args.nenv)
+    off, R, sp = synth.workload_envs(args.config, nenv=args.nenv or None)
     ev = HIPEvaluator(basis, c, device=0)
     b = EnvBatch(off, R.T.copy(), sp, device=dev)
-    E = torch.empty(b.nenv, dtype=torch.float64, device=dev)
-    dE = torch.empty((3, b.nnbr), dtype=torch.float64, device=dev)
+    nc = 1 if ev.prop == 0 else 3
+    E = torch.empty((b.nenv,) if nc == 1 else (b.nenv, 3), dtype=torch.float64, device=dev)
+    dE = torch.empty((3 * nc, b.nnbr), dtype=torch.float64, device=dev)
     ev.energy_forces(b, E, dE)            # validated call
     torch.cuda.synchronize()
     ev.set_validation(False)              # inputs validated once above
@@ -281,6 +313,7 @@ def main():
         torch.cuda.synchronize()
 
     pk = ev.pack
+    dag_parts, dag_tail, dag_kernel = ev.last_dag_shape()
     f_alg = algorithmic_flops(pk, True)
     f_exe = executed_flops(pk, True)
     dag_avg = float(np.mean(dag_ms))
@@ -299,9 +332,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], "
-                               "order 4, degree 12, Lmax 6, Invariant, FP64",
-                   "nenv": b.nenv, "nnbr": b.nnbr, "order": 4, "maxdeg": 12, "L": 6,
+        "config": {"workload": WORKLOAD_NAMES[args.config] if b.nenv == w["nenv"] else
+                   f"{WORKLOAD_NAMES[args.config]} (run at {b.nenv} environments)",
+                   "nenv": b.nenv, "nnbr": b.nnbr, "order": w["order"], "maxdeg": w["maxdeg"], "L": w["L"],
                    "len_B": len(basis), "len_AA": len(pk.orders), "npool": len(pk.pool),
                    "tile_waves": ev.waves,
                    # 0: chosen per call (parts from the tile and CU counts, and
@@ -309,12 +342,13 @@ def main():
                    "tile_parts": ("automatic" if tuned is None or tuned[0][1] == 0 else tuned[0][1]),
                    "autotune_ms": None if tuned is None else {f"{k[0]}x{k[1] or 'auto'}": round(v, 3)
                                                               for k, v in tuned[1].items()}},
-        "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (no MFMA); MI355X FP64 peak",
+        "roofline": {"bound": "fp64-vector", "compute": "FP64 vector ALU (no MFMA); MI355X FP64 vector peak",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": DAG_TRAFFIC_BYTES.get(b.nenv),
-                     "traffic_source": TRAFFIC_SOURCE if b.nenv in DAG_TRAFFIC_BYTES else None,
-                     "kernel": "k_dagr", "kernel_ms": round(dag_avg, 4),
+                     "traffic": DAG_TRAFFIC_BYTES.get(b.nenv) if headline else None,
+                     "traffic_source": TRAFFIC_SOURCE if headline and b.nenv in DAG_TRAFFIC_BYTES else None,
+                     "kernel": "k_dagr" if dag_kernel == 0 else f"k_dag ({dag_kernel}-environment tiles)",
+                     "kernel_ms": round(dag_avg, 4), "dag_parts": dag_parts, "dag_tail_tiles": dag_tail,
                      "flops_per_env_algorithmic": f_alg, "flops_per_env_executed": f_exe,
                      "executed_tflops": round(achieved_exe, 3),
                      "executed_frac": round(achieved_exe / FP64_PEAK_TFLOPS, 4),
@@ -322,17 +356,19 @@ def main():
                      "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
                      "force_ms": round(float(np.mean(force_ms)), 4)},
     }
-    if not args.no_md:
+    if not headline:
+        out["metric"] = "env/s energy+forces, plain FP64 (configuration profile run)"
+    if not args.no_md and headline:
         out["md_step"] = md_leg(ev, args.steps, args.warmup)
-    if not args.no_surrogate:
+    if not args.no_surrogate and headline:
         E_plain = E.clone()
         sur = surrogate_leg(ev, b, E_plain, args.steps, args.warmup, args.train_envs, args.eval_envs,
                             args.epochs, log=lambda m: print("[surrogate]", m, file=sys.stderr, flush=True))
         sur["speedup_vs_plain"] = round(sur["value"] / value, 3)
         out["surrogate"] = sur
-    if not args.no_cpu:
+    if not args.no_cpu and ev.prop == 0:
         T = basis_tables(basis)
-        out["cpu_baseline"] = cpu_baseline(T, c, off, R)
+        out["cpu_baseline"] = cpu_baseline(T, c, off, R, sp)
     print(json.dumps(out), flush=True)
 
 
