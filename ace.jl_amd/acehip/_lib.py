@@ -52,6 +52,7 @@ class AcePack(C.Structure):
         ("ntasks", C.c_int32), ("tasks", _i32p), ("task_term_ptr", _i32p),
         ("nterms", C.c_int32), ("term_row", _i32p), ("term_scale", _f64p), ("term_conj", _i32p),
         ("ngslots", C.c_int32), ("gslot_ptr", _i32p), ("gslot_list", _i32p),
+        ("classed", C.c_int32), ("gslot_class", _i32p),
         ("nrec", C.c_int32), ("rec_hdr", C.POINTER(C.c_uint32)), ("rec_leaf", _i32p),
         ("nfeat", C.c_int32), ("feat_rows", _i32p), ("feat_ptr", _i32p), ("feat_f", _i32p),
         ("feat_coef", _f64p), ("fgrad_ptr", _i32p), ("fgrad_feat", _i32p), ("fgrad_f", _i32p),
@@ -62,7 +63,7 @@ class AcePack(C.Structure):
 
 
 # every symbol declared in include/acehip.h
-EXPORTS = ("ace_ctx_create", "ace_ctx_create_jl", "ace_compile_check", "ace_last_dag_shape", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_set_tail_parts", "ace_energy",
+EXPORTS = ("ace_ctx_create", "ace_ctx_create_jl", "ace_compile_check", "ace_last_dag_shape", "ace_set_kernel", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_set_tail_parts", "ace_energy",
            "ace_energy_forces", "ace_basis_A", "ace_basis_AA", "ace_basis_B",
            "ace_basis_dB", "ace_surr_features", "ace_surr_energy_forces",
            "ace_last_timing", "ace_set_validation", "ace_last_error",
@@ -109,6 +110,7 @@ def load(path: str | None = None):
                                            C.c_double, vp, C.c_double, vp, vp, vp, vp, vp, vp]
     lib.ace_last_timing.argtypes = [vp, _f64p, _f64p, _f64p]
     lib.ace_set_validation.argtypes = [vp, C.c_int]
+    lib.ace_set_kernel.argtypes = [vp, C.c_int]
     lib.ace_last_error.restype = C.c_char_p
     lib.ace_last_error.argtypes = []
     lib.ace_ctx_destroy.argtypes = [vp]
@@ -198,6 +200,8 @@ def make_pack(pk):
     s.ngslots = pk.ngslots
     s.gslot_ptr = _ptr(arr(pk.gslot_ptr, np.int32), C.c_int32)
     s.gslot_list = _ptr(arr(pk.gslot_list, np.int32), C.c_int32)
+    s.classed = int(pk.classed)
+    s.gslot_class = _ptr(arr(pk.gslot_class, np.int32), C.c_int32)
     s.nrec = len(pk.rec_hdr)
     s.rec_hdr = _ptr(arr(pk.rec_hdr, np.uint32), C.c_uint32)
     s.rec_leaf = _ptr(arr(pk.rec_leaf, np.int32), C.c_int32)

@@ -106,7 +106,8 @@ class HIPEvaluator:
 
     def last_dag_shape(self):
         """Product-DAG launch of the last call: (parts per tile, tail tiles,
-        kernel: 0 records, else the run interpreter's environments per tile)."""
+        kernel: 0 / 1 the record kernel on 64 / 32-environment tiles, else the
+        run interpreter's environments per tile)."""
         p, t, k = C.c_int(), C.c_int64(), C.c_int()
         _lib.check(self.lib.ace_last_dag_shape(self.ctx, C.byref(p), C.byref(t), C.byref(k)))
         return p.value, t.value, k.value
@@ -124,6 +125,11 @@ class HIPEvaluator:
     def set_parts(self, parts: int = 0):
         """Workgroups per product-DAG tile: 0 chooses per call (default), else 1 to 4."""
         _lib.check(self.lib.ace_set_parts(self.ctx, int(parts)))
+
+    def set_kernel(self, mode: int = 0):
+        """Product-DAG kernel: 0 the record kernel where the basis allows it
+        (default), 1 the run interpreter always."""
+        _lib.check(self.lib.ace_set_kernel(self.ctx, int(mode)))
 
     def set_tail_parts(self, ntail: int = -1):
         """Tiles at the end of the product-DAG grid that run one more part:

@@ -33,28 +33,54 @@ class Pack:
         if iAA2iA.shape[1] > P.MAX_ORDER:
             raise ValueError("correlation order above 8 is not supported")
         is_m0 = self.pool[:, 3] == 0
+        # A2B CSR (rows ascending, columns ascending within a row)
+        a2b = T["a2b"]
+        rows_ = np.asarray(a2b["rows"], dtype=np.int64)
+        cols_ = np.asarray(a2b["cols"], dtype=np.int64)
+        vals_ = np.asarray(a2b["vals"])
+        order = np.lexsort((cols_, rows_))
+        rows_, cols_, vals_ = rows_[order], cols_[order], vals_[order]
+        # EuclideanVector: class-split programs when the A2B map allows it
+        self.classed = 0
+        self.row_class = P.row_classes(orders, iAA2iA, aspec)
+        if self.prop == 1 and P.a2b_is_classed(cols_, vals_, self.row_class):
+            self.classed = 1
         poly, _ = P.energy_polynomial(orders, iAA2iA, kk, cj, sgn)
         poly = P.merge_conjugates(poly, is_m0)
         self.poly = poly
         G = P.gradient_polynomials(poly, len(self.pool), is_m0)
         prog = P.RunProgram()
-        self.energy_tasks = [prog.add_task(part, -1) for part in P.split_poly(poly, ENERGY_PARTS)]
+        if self.classed:
+            # energy: one task group per class Q_M, output slot -1 - class
+            self.energy_tasks = []
+            for c, q in enumerate(P.split_by_class(poly, self.row_class)):
+                if q:
+                    self.energy_tasks += [prog.add_task(part, -1 - c) for part in P.split_poly(q, ENERGY_PARTS)]
+        else:
+            self.energy_tasks = [prog.add_task(part, -1) for part in P.split_poly(poly, ENERGY_PARTS)]
         # gradient tasks: split large polynomials over several output slots
+        # (and, class-split programs, by class: slot class in gslot_class)
         gslots = [[] for _ in range(len(self.pool))]
+        gclass = []
         self.grad_tasks = []
         nslot = 0
         for k, g in enumerate(G):
             if not g:
                 continue
-            cost = sum(1.0 for m in g) + 0.0
-            nparts = max(1, int(np.ceil(cost / MAX_TASK_COST)))
-            parts = P.split_poly(g, nparts) if nparts > 1 else [g]
-            for part in parts:
-                t = prog.add_task(part, nslot)
-                self.grad_tasks.append(t)
-                gslots[k].append(nslot)
-                nslot += 1
+            for c, gc in (enumerate(P.split_by_class(g, self.row_class)) if self.classed else [(0, g)]):
+                if not gc:
+                    continue
+                cost = sum(1.0 for m in gc) + 0.0
+                nparts = max(1, int(np.ceil(cost / MAX_TASK_COST)))
+                parts = P.split_poly(gc, nparts) if nparts > 1 else [gc]
+                for part in parts:
+                    t = prog.add_task(part, nslot)
+                    self.grad_tasks.append(t)
+                    gslots[k].append(nslot)
+                    gclass.append(c)
+                    nslot += 1
         self.ngslots = nslot
+        self.gslot_class = np.asarray(gclass if self.classed else [], dtype=np.int32)
         self.gslot_ptr = np.zeros(len(self.pool) + 1, dtype=np.int32)
         self.gslot_ptr[1:] = np.cumsum([len(s) for s in gslots])
         self.gslot_list = np.asarray([s for ss in gslots for s in ss], dtype=np.int32)
@@ -63,7 +89,7 @@ class Pack:
         self.runs = np.asarray(prog.runs, dtype=np.int32).reshape(-1, P.RUN_W)
         self.leaf_fid = np.asarray(prog.leaf_fid, dtype=np.int32)
         tasks3 = np.asarray(prog.tasks, dtype=np.int32).reshape(-1, 3)
-        self._build_records(tasks3, records=iAA2iA.shape[1] <= 4)
+        self._build_records(tasks3, records=iAA2iA.shape[1] <= 4 and (self.prop == 0 or self.classed == 1))
         lt_ptr, rows, scales, conjs = [0], [], [], []
         for terms in prog.leaf_terms:
             for (r, s, c) in terms:
@@ -105,13 +131,6 @@ class Pack:
         self.wave_task_list = np.asarray(wtl, dtype=np.int32)
         self.wave_etask_ptr = np.asarray(wep, dtype=np.int32)
         self.wave_etask_list = np.asarray(wel, dtype=np.int32)
-        # A2B CSR
-        a2b = T["a2b"]
-        rows_ = np.asarray(a2b["rows"], dtype=np.int64)
-        cols_ = np.asarray(a2b["cols"], dtype=np.int64)
-        vals_ = np.asarray(a2b["vals"])
-        order = np.lexsort((cols_, rows_))
-        rows_, cols_, vals_ = rows_[order], cols_[order], vals_[order]
         nB, nAA = a2b["shape"]
         self.nB, self.nAA = int(nB), int(nAA)
         rp = np.zeros(self.nB + 1, dtype=np.int64)

@@ -140,6 +140,83 @@ def gradient_polynomials(poly, npool, is_m0):
     return [dict(g) for g in G]
 
 
+def row_classes(orders, iAA2iA, aspec):
+    """Class index M + 1 of every AA row, M = the sum of its factors' m (the
+    total m of the product); -1 for the order-0 row."""
+    out = np.full(len(orders), -1, dtype=np.int64)
+    m = np.asarray(aspec)[:, 3]
+    for i in range(len(orders)):
+        o = int(orders[i])
+        if o:
+            out[i] = int(m[iAA2iA[i, :o]].sum()) + 1
+    return out
+
+
+def class_vectors(cols, vals3, row_class, tol=1e-12):
+    """Class structure of a EuclideanVector A2B map: ``(ok, u, sig)``.
+
+    Every value of an AA column of class c (= M + 1, M the column's total m)
+    should be one complex number times a unit vector u[c], with conj(u[c]) =
+    sig[c] u[2 - c] (the l = 1 coupling of symmbasis.jl:267-285).  u[c] is the
+    value of largest norm of the class (the first such), normalised; a class
+    without entries takes conj(u) of its mirror.  ``ok``: every entry within
+    tol * (largest norm of the map) of its class line and |sig[c]| >= 1 - tol
+    where both classes occur.  ``vals3`` [nnz, 3] complex, in CSR order."""
+    cls = np.asarray(row_class)[np.asarray(cols)]
+    u = np.zeros((3, 3), dtype=np.complex128)
+    sig = np.zeros(3, dtype=np.complex128)
+    if np.any((cls < 0) | (cls > 2)):
+        return False, u, sig
+    V = np.asarray(vals3, dtype=np.complex128).reshape(-1, 3)
+    nrm = np.sqrt(np.sum(V.real ** 2 + V.imag ** 2, axis=1))
+    maxn = float(nrm.max()) if len(nrm) else 0.0
+    have = [False] * 3
+    for c in range(3):
+        idx = np.nonzero(cls == c)[0]
+        if len(idx) and nrm[idx].max() > 0:
+            z = idx[int(np.argmax(nrm[idx]))]
+            u[c] = V[z] / nrm[z]
+            have[c] = True
+    for c in range(3):
+        if not have[c] and have[2 - c]:
+            u[c] = np.conj(u[2 - c])
+    U = u[cls]
+    d = np.sum(V * np.conj(U), axis=1)
+    R = V - d[:, None] * U
+    if len(R) and np.sqrt(np.sum(R.real ** 2 + R.imag ** 2, axis=1)).max() > tol * maxn:
+        return False, u, sig
+    for c in range(3):
+        sig[c] = np.conj(np.sum(u[c] * u[2 - c]))
+        if have[c] and have[2 - c] and abs(sig[c]) < 1.0 - tol:
+            return False, u, sig
+    return True, u, sig
+
+
+def a2b_is_classed(cols, vals3, row_class, tol=1e-12):
+    """True when the EuclideanVector A2B map has the class structure of
+    :func:`class_vectors`: then the three Cartesian outputs are fixed
+    combinations of three scalar-coefficient polynomials Q_M."""
+    return class_vectors(cols, vals3, row_class, tol)[0]
+
+
+def term_class(term, row_class):
+    """Class of a symbolic coefficient term (row, scale, conj): the row's
+    class, mirrored (M -> -M) when the term is conjugated."""
+    row, _, cj = term
+    c = int(row_class[row])
+    return 2 - c if cj else c
+
+
+def split_by_class(poly, row_class):
+    """The polynomial as three polynomials, one per term class (insertion
+    order kept)."""
+    out = [dict(), dict(), dict()]
+    for mono, terms in poly.items():
+        for t in terms:
+            out[term_class(t, row_class)].setdefault(mono, []).append(t)
+    return out
+
+
 def eval_terms(terms, ct):
     """Numeric value of a symbolic coefficient for c~ (real or (nAA,3))."""
     v = 0.0

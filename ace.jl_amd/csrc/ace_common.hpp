@@ -222,6 +222,16 @@ struct ace_ctx {
     int npool = 0, nA = 0, nAA = 0, maxorder = 0, nB = 0, nnz = 0;
     int nruns = 0, nleaves = 0, ntasks = 0, nterms = 0, ngslots = 0;
     int ncoef = 1;
+    // class-split EuclideanVector programs: tasks accumulate one complex
+    // polynomial of class c = M + 1, G slots hold one complex value, and the
+    // Cartesian components are sum_c u[c][p] (class value)
+    int classed = 0;
+    // record kernel tile: 64 or 32 environments (0: the run interpreter only);
+    // force_interp: run the interpreter even so (ace_set_kernel)
+    int rec_et = 0;
+    bool force_interp = false;
+    DevBuf<double2> cls_u;        // [3 classes][3 components]
+    DevBuf<int32_t> gslot_cls;    // [ngslots] class of every G slot
     acehost::PackCopy host;
     // device
     DevBuf<double> rA, rB, rC;
@@ -240,7 +250,7 @@ struct ace_ctx {
     DevBuf<int32_t> gfold;
     DevBuf<int32_t> runs, leaf_fid, d_tasks, gslot_ptr, gslot_list;
     DevBuf<double> leaf_coef, task_const, e0v;
-    DevBuf<double4> vleaf;  // EuclideanVector decoded leaves
+    DevBuf<double4> vleaf;  // EuclideanVector decoded leaves (two per leaf; one when class-split)
     DevBuf<uint32_t> recs;
     // surrogate feature tables
     int nfeat = 0;
@@ -289,8 +299,12 @@ struct ace_ctx {
     }
 };
 
+// Complex values per G slot: the Cartesian components (3) of a
+// EuclideanVector basis, or one (Invariant, class-split programs).
+inline int gslot_width(const ace_ctx* ctx) { return ctx->prop == ACE_PROP_INVARIANT || ctx->classed ? 1 : 3; }
+
 // Device scratch of one evaluation call: the pooled values A+
-// [npool][estride], the adjoint G [ngslots * NC][estride], the per-part
+// [npool][estride], the adjoint G [ngslots * gslot_width][estride], the per-part
 // energy shares [parts][estride] and the neighbour -> environment map of the
 // flat force kernel [nnbr].  Either the context's own buffers or carved from
 // the allocation-free entry point's fixed scratch array.

@@ -17,6 +17,32 @@ namespace {
 // ---------------------------------------------------------------------------
 // k_force: +dE/dr_j = Re sum_k G[k] grad(R_n Y_lm)(r_j)
 
+// Component p of G[k] for one environment: the sum over k's slots.  With
+// class-split programs (gcls set) a slot holds one complex value of class
+// gcls[slot], and component p is sum_slots cls_u[class][p] * value.
+template <int NC>
+__device__ __forceinline__ double2 gather_g(const int32_t* __restrict__ gslot_ptr,
+                                            const int32_t* __restrict__ gslot_list,
+                                            const int32_t* __restrict__ gcls, const double2* __restrict__ cls_u,
+                                            const double2* __restrict__ Gbuf, int64_t estride, int64_t env, int k,
+                                            int p) {
+    double2 g = make_double2(0.0, 0.0);
+    for (int s = gslot_ptr[k]; s < gslot_ptr[k + 1]; ++s) {
+        const int slot = gslot_list[s];
+        if (NC > 1 && gcls) {
+            const double2 v = Gbuf[(int64_t)slot * estride + env];
+            const double2 u = cls_u[3 * gcls[slot] + p];
+            g.x += u.x * v.x - u.y * v.y;
+            g.y += u.x * v.y + u.y * v.x;
+        } else {
+            const double2 v = Gbuf[((int64_t)slot * NC + p) * estride + env];
+            g.x += v.x;
+            g.y += v.y;
+        }
+    }
+    return g;
+}
+
 template <int L, int PROP>
 __global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
                                                const int32_t* __restrict__ off,
@@ -25,6 +51,8 @@ __global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
                                                const int16_t* __restrict__ kidx, int npool,
                                                const int32_t* __restrict__ gslot_ptr,
                                                const int32_t* __restrict__ gslot_list,
+                                               const int32_t* __restrict__ gcls,
+                                               const double2* __restrict__ cls_u,
                                                const double2* __restrict__ Gbuf, int64_t estride,
                                                double* __restrict__ dE) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
@@ -38,13 +66,7 @@ __global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
     if (env >= nenv) return;
     for (int q = lane; q < npool * NC; q += 64) {
         const int k = q / NC, p = q % NC;
-        double2 g = make_double2(0.0, 0.0);
-        for (int s = gslot_ptr[k]; s < gslot_ptr[k + 1]; ++s) {
-            const double2 v = Gbuf[((int64_t)gslot_list[s] * NC + p) * estride + env];
-            g.x += v.x;
-            g.y += v.y;
-        }
-        Gs[q] = g;
+        Gs[q] = gather_g<NC>(gslot_ptr, gslot_list, gcls, cls_u, Gbuf, estride, env, k, p);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -163,6 +185,8 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
                                                const int32_t* __restrict__ pool, int npool,
                                                const int32_t* __restrict__ gslot_ptr,
                                                const int32_t* __restrict__ gslot_list,
+                                               const int32_t* __restrict__ gcls,
+                                               const double2* __restrict__ cls_u,
                                                const double2* __restrict__ Gbuf, int64_t estride,
                                                double* __restrict__ dE) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
@@ -179,15 +203,7 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
         const int mu = max(0, pool[4 * k]), n = pool[4 * k + 1] - 1, l = pool[4 * k + 2], m = pool[4 * k + 3];
         const int d = ((mu * NP + idxp(l, m)) * nr + n) * NC;
 #pragma unroll
-        for (int p = 0; p < NC; ++p) {
-            double2 g = make_double2(0.0, 0.0);
-            for (int s = gslot_ptr[k]; s < gslot_ptr[k + 1]; ++s) {
-                const double2 v = Gbuf[((int64_t)gslot_list[s] * NC + p) * estride + env];
-                g.x += v.x;
-                g.y += v.y;
-            }
-            Gd[d + p] = g;
-        }
+        for (int p = 0; p < NC; ++p) Gd[d + p] = gather_g<NC>(gslot_ptr, gslot_list, gcls, cls_u, Gbuf, estride, env, k, p);
     }
     __syncthreads();
     const int j0 = off[env], j1 = off[env + 1];
@@ -546,6 +562,7 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
         return;
     }
     const int nsp = std::max(1, ctx->nspecies);
+    const int32_t* gcls = ctx->classed ? ctx->gslot_cls.p : nullptr;
     const size_t dense = (size_t)nsp * NP * ctx->nr * NC * sizeof(double2);
     if (ctx->nr <= FORCE_NRMAX && dense <= 64 * 1024) {
         ForceTab ft;
@@ -553,14 +570,15 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
         set_func_lds((const void*)k_force<L, PROP>, dense);
         hipLaunchKernelGGL((k_force<L, PROP>), dim3((unsigned)nenv), dim3(64), dense, st, radial_params(ctx), ft,
                            nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, nsp, ctx->pool.p,
-                           ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ws.Gbuf, estride, dE);
+                           ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, gcls, ctx->cls_u.p, ws.Gbuf, estride,
+                           dE);
         return;
     }
     const size_t lds = (size_t)ctx->npool * NC * sizeof(double2) + 64 * 2 * ctx->nr * sizeof(double);
     set_func_lds((const void*)k_force_lds<L, PROP>, lds);
     hipLaunchKernelGGL((k_force_lds<L, PROP>), dim3((unsigned)nenv), dim3(64), lds, st, radial_params(ctx),
                        nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, ctx->kidx.p,
-                       ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, ws.Gbuf, estride, dE);
+                       ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, gcls, ctx->cls_u.p, ws.Gbuf, estride, dE);
 }
 
 }  // namespace

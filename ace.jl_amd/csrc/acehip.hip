@@ -30,7 +30,8 @@ struct DagArgs {
     const int32_t* runs;       // [nruns][ACE_RUN_W]
     const int32_t* leaf_fid;   // [nleaves]
     const double* leaf_coef;   // [nleaves][ncoef]
-    const double4* vleaf;      // EuclideanVector: [nleaves][2] decoded leaves (factor id, then 3 complex coefs)
+    const double4* vleaf;      // EuclideanVector: [nleaves][2] decoded leaves (factor id, then 3 complex coefs);
+                               // class-split: [nleaves] (factor id, 0, one complex coef)
     const int32_t* tasks;      // [ntasks][3]
     const double* task_const;  // [ntasks][2 | 6]
     const int32_t* wtask_ptr;  // [W+1]
@@ -42,6 +43,9 @@ struct DagArgs {
 
 // PROP 0: real coefficient, output Re(acc) (energy) or acc (G slot).
 // PROP 1: 3 complex coefficients, output Re(acc_p) or acc_p.
+// PROP 2: class-split EuclideanVector programs: one complex coefficient; an
+//         energy task (slot -1 - c) adds acc to the class value Q_c, the
+//         energy is E_p = e0_p + Re sum_c u[c][p] Q_c; a G slot stores acc.
 // ENVS = 64: lanes = 64 environments, one task stream per wave.
 // ENVS = 32 / 16: the lane groups of a wave are the same ENVS environments
 //            running 64 / ENVS task streams (smaller LDS tiles for large
@@ -52,8 +56,10 @@ template <int PROP, int ENVS, int MAXD>
 __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
                                               const double2* __restrict__ Aplus,
                                               int64_t estride, double* __restrict__ E_out,
-                                              double2* __restrict__ Gbuf, const double* __restrict__ e0v) {
-    constexpr int NC = PROP == 0 ? 1 : 3;
+                                              double2* __restrict__ Gbuf, const double* __restrict__ e0v,
+                                              const double2* __restrict__ cls_u) {
+    constexpr int NC = PROP == 1 ? 3 : 1;             // complex accumulators per task
+    constexpr int NE = PROP == 0 ? 1 : PROP == 1 ? 3 : 6;  // energy partials per lane
     constexpr int SPW = 64 / ENVS;  // task streams per wave
     extern __shared__ double2 As[];  // [npool][ENVS]
     const int lane = threadIdx.x & 63;
@@ -67,9 +73,9 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
         As[k * ENVS + el] = valid ? Aplus[(int64_t)k * estride + env] : make_double2(0.0, 0.0);
     __syncthreads();
 
-    double ep[NC];
+    double ep[NE];
 #pragma unroll
-    for (int p = 0; p < NC; ++p) ep[p] = 0.0;
+    for (int q = 0; q < NE; ++q) ep[q] = 0.0;
 
     const int tb = a.wtask_ptr[stream], te = a.wtask_ptr[stream + 1];
     for (int ti = tb; ti < te; ++ti) {
@@ -107,6 +113,15 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
                     S[0].x += c * v.x;
                     S[0].y += c * v.y;
                 }
+            } else if (PROP == 2) {
+                // one 32-byte decoded leaf: {factor id (low word), 0, c.re, c.im}
+#pragma unroll 2
+                for (int i = 0; i < nl; ++i) {
+                    const double4 h = a.vleaf[lo + i];
+                    const double2 v = lds_factor<ENVS>(As, (int)__double2loint(h.x), el);
+                    S[0].x += h.z * v.x - h.w * v.y;
+                    S[0].y += h.z * v.y + h.w * v.x;
+                }
             } else {
                 // one 64-byte decoded leaf per iteration: {factor id, 0, c0, c1}, {c2, c3, c4, c5};
                 // unrolled by 2 so that consecutive leaf fetches overlap
@@ -129,8 +144,18 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
             }
         }
         if (slot < 0) {
+            if (PROP == 2) {
+                const int c = -1 - slot;
 #pragma unroll
-            for (int p = 0; p < NC; ++p) ep[p] += acc[p].x;
+                for (int q = 0; q < 3; ++q)
+                    if (c == q) {
+                        ep[2 * q] += acc[0].x;
+                        ep[2 * q + 1] += acc[0].y;
+                    }
+            } else {
+#pragma unroll
+                for (int p = 0; p < NC; ++p) ep[p] += acc[p].x;
+            }
         } else if (valid) {
 #pragma unroll
             for (int p = 0; p < NC; ++p) Gbuf[((int64_t)slot * NC + p) * estride + env] = acc[p];
@@ -140,30 +165,60 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
     __syncthreads();
     double* red = reinterpret_cast<double*>(As);
 #pragma unroll
-    for (int p = 0; p < NC; ++p) red[(stream * NC + p) * ENVS + el] = ep[p];
+    for (int q = 0; q < NE; ++q) red[(stream * NE + q) * ENVS + el] = ep[q];
     __syncthreads();
     if (wave == 0 && lane < ENVS && valid && E_out) {
+        if (PROP == 2) {
+            double Q[6];
 #pragma unroll
-        for (int p = 0; p < NC; ++p) {
-            double sum = e0v[p];
-            for (int w = 0; w < nwaves * SPW; ++w) sum += red[(w * NC + p) * ENVS + el];
-            E_out[env * NC + p] = sum;
+            for (int q = 0; q < 6; ++q) {
+                Q[q] = 0.0;
+                for (int w = 0; w < nwaves * SPW; ++w) Q[q] += red[(w * 6 + q) * ENVS + el];
+            }
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                double sum = e0v[p];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 u = cls_u[3 * c + p];
+                    sum += u.x * Q[2 * c] - u.y * Q[2 * c + 1];
+                }
+                E_out[env * 3 + p] = sum;
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < NC; ++p) {
+                double sum = e0v[p];
+                for (int w = 0; w < nwaves * SPW; ++w) sum += red[(w * NC + p) * ENVS + el];
+                E_out[env * NC + p] = sum;
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_dagr: the same programs as 64-byte decoded records (Invariant).  A
-// record holds one run (or the continuation of a long run): up to four leaves
-// with their coefficients and the prefix levels it recomputes, all as ready
-// LDS byte offsets, so the loop spends few scalar instructions per record.
+// k_dagr: the same programs as decoded records (Invariant, class-split
+// EuclideanVector).  A record holds one run (or the continuation of a long
+// run): up to four leaves with their coefficients and the prefix levels it
+// recomputes, all as ready LDS byte offsets, so the loop spends few scalar
+// instructions per record.
+//   CPLX = false: real leaf coefficients, 64-byte records, one energy value.
+//   CPLX = true:  one complex coefficient per leaf, 96-byte records; energy
+//                 tasks (slot -1 - c) add to the class value Q_c and
+//                 E_p = e0_p + Re sum_c u[c][p] Q_c.
+//   HALF = false: 64 environments per tile, lane = environment.
+//   HALF = true:  32 environments per tile (pooled bases of 161-320 values):
+//                 lanes e and e + 32 hold environment e and run the same
+//                 record stream, the low half on leaves 0-1, the high half on
+//                 leaves 2-3; the halves' sums meet at the end of each task.
 
 struct DagRecArgs {
-    const uint32_t* recs;      // [nrec + 1][16]
+    const uint32_t* recs;      // [nrec + 1][16 | 24]
     const int32_t* tasks;      // [ntasks][5]
     const double* task_const;  // [ntasks][2]
     const int32_t* wtask_ptr;  // [parts * W + 1]
     const int32_t* wtask_list;
+    const double2* cls_u;      // CPLX: [3 classes][3 components]
     int npool;
     int W;
     int parts;                 // workgroups per tile, each running 1/parts of the tasks
@@ -174,42 +229,68 @@ struct DagRecArgs {
     const int32_t* wtask_list_hi;
 };
 
-
-
-__device__ __forceinline__ double2 lds_conj(const double2* As, uint32_t f, int lane) {
-    double2 v = As[(f & 0x7fffu) * 64 + lane];
-    v.y = flip_sign_if(v.y, f);
-    return v;
-}
-
 __device__ __forceinline__ double rec_coef(uint32_t lo, uint32_t hi) {
     return __hiloint2double((int)hi, (int)lo);
 }
 
+// c * v for a complex coefficient held as a uint4 (re lo, re hi, im lo, im hi)
+__device__ __forceinline__ void rec_cfma(double2& S, const uint4& c, const double2& v) {
+    const double cr = rec_coef(c.x, c.y), ci = rec_coef(c.z, c.w);
+    S.x += cr * v.x - ci * v.y;
+    S.y += cr * v.y + ci * v.x;
+}
+
 // One record of a given type (MORE: leaves 2 and 3 present; U1 / U2 / U3:
 // prefix levels recomputed): its LDS reads first, then its arithmetic.
-template <bool MORE, bool U1, bool U2, bool U3>
+// h2..h5: the coefficients (real: h2 = c0 c1, h3 = c2 c3; complex: one each).
+template <bool MORE, bool U1, bool U2, bool U3, bool CPLX, bool HALF>
 __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const uint4& h0, const uint4& h1,
-                                            const uint4& h2, const uint4& h3, double2& acc, double2& P1,
-                                            double2& P2, double2& Pc) {
+                                            const uint4& h2, const uint4& h3, const uint4& h4, const uint4& h5,
+                                            bool hi, double2& acc, double2& P1, double2& P2, double2& Pc) {
     auto at = [&](uint32_t off) { return *reinterpret_cast<const double2*>(lds + off); };
-    const double2 v0 = at(h0.y), v1 = at(h0.z);
-    double2 v2, v3, a1, a2, a3;
-    if (MORE) {
-        v2 = at(h0.w);
-        v3 = at(h1.x);
+    double2 v0, v1, v2, v3, a1, a2, a3;
+    if (HALF) {
+        // absent leaves have offset 0 and coefficient 0
+        v0 = at(hi ? h0.w : h0.y);
+        v1 = at(hi ? h1.x : h0.z);
+    } else {
+        v0 = at(h0.y);
+        v1 = at(h0.z);
+        if (MORE) {
+            v2 = at(h0.w);
+            v3 = at(h1.x);
+        }
     }
     if (U1) a1 = at(h1.y & 0x7fffffffu);
     if (U2) a2 = at(h1.z & 0x7fffffffu);
     if (U3) a3 = at(h1.w & 0x7fffffffu);
-    const double c0 = rec_coef(h2.x, h2.y), c1 = rec_coef(h2.z, h2.w);
-    double2 S;
-    S.x = c0 * v0.x + c1 * v1.x;
-    S.y = c0 * v0.y + c1 * v1.y;
-    if (MORE) {
-        const double c2 = rec_coef(h3.x, h3.y), c3 = rec_coef(h3.z, h3.w);
-        S.x += c2 * v2.x + c3 * v3.x;
-        S.y += c2 * v2.y + c3 * v3.y;
+    double2 S = make_double2(0.0, 0.0);
+    if (CPLX) {
+        if (HALF) {
+            rec_cfma(S, hi ? h4 : h2, v0);
+            rec_cfma(S, hi ? h5 : h3, v1);
+        } else {
+            rec_cfma(S, h2, v0);
+            rec_cfma(S, h3, v1);
+            if (MORE) {
+                rec_cfma(S, h4, v2);
+                rec_cfma(S, h5, v3);
+            }
+        }
+    } else if (HALF) {
+        const double c0 = hi ? rec_coef(h3.x, h3.y) : rec_coef(h2.x, h2.y);
+        const double c1 = hi ? rec_coef(h3.z, h3.w) : rec_coef(h2.z, h2.w);
+        S.x = c0 * v0.x + c1 * v1.x;
+        S.y = c0 * v0.y + c1 * v1.y;
+    } else {
+        const double c0 = rec_coef(h2.x, h2.y), c1 = rec_coef(h2.z, h2.w);
+        S.x = c0 * v0.x + c1 * v1.x;
+        S.y = c0 * v0.y + c1 * v1.y;
+        if (MORE) {
+            const double c2 = rec_coef(h3.x, h3.y), c3 = rec_coef(h3.z, h3.w);
+            S.x += c2 * v2.x + c3 * v3.x;
+            S.y += c2 * v2.y + c3 * v3.y;
+        }
     }
     S.y = flip_sign_if(S.y, ctl >> 16);  // conjugated leaves: conj(sum c A) = sum c conj(A)
     if (U1) {
@@ -230,64 +311,84 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
     acc.y += Pc.x * S.y + Pc.y * S.x;
 }
 
+template <bool CPLX, bool HALF>
 __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
                                                const double2* __restrict__ Aplus,
                                                int64_t estride, double* __restrict__ E_out,
                                                double2* __restrict__ Gbuf, const double* __restrict__ e0p,
                                                const int32_t* __restrict__ env_list,
                                                const int32_t* __restrict__ env_count) {
-    extern __shared__ double2 As[];  // [npool][64]
+    constexpr int ET = HALF ? 32 : 64;     // environments per tile
+    constexpr int RQ = CPLX ? 6 : 4;       // uint4 per record
+    constexpr int NE = CPLX ? 6 : 1;       // energy partials per lane
+    extern __shared__ double2 As[];  // [npool][ET]
     const int lane = threadIdx.x & 63;
+    const int el = HALF ? (lane & 31) : lane;
+    const bool hi = HALF && lane >= 32;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // workgroup = (tile, part): the parts of a tile run disjoint task lists
     const int64_t blo = a.ntile_lo * a.parts;
-    const bool hi = (int64_t)blockIdx.x >= blo;
-    const int P = hi ? a.parts + 1 : a.parts;
-    const int64_t bt = hi ? (int64_t)blockIdx.x - blo : (int64_t)blockIdx.x;
-    const int64_t tile = (hi ? a.ntile_lo : 0) + bt / P;
+    const bool tail = (int64_t)blockIdx.x >= blo;
+    const int P = tail ? a.parts + 1 : a.parts;
+    const int64_t bt = tail ? (int64_t)blockIdx.x - blo : (int64_t)blockIdx.x;
+    const int64_t tile = (tail ? a.ntile_lo : 0) + bt / P;
     const int part = (int)(bt % P);
-    const int32_t* const wptr = hi ? a.wtask_ptr_hi : a.wtask_ptr;
-    const int32_t* const wlist = hi ? a.wtask_list_hi : a.wtask_list;
-    int64_t env = tile * 64 + lane;
+    const int32_t* const wptr = tail ? a.wtask_ptr_hi : a.wtask_ptr;
+    const int32_t* const wlist = tail ? a.wtask_list_hi : a.wtask_list;
+    int64_t env = tile * ET + el;
     bool valid = env < nenv;
     if (env_list) {
         // environments listed by the caller (fallback of the learned surrogate)
         const int64_t cnt = *env_count;
-        if (tile * 64 >= cnt) return;  // whole workgroup idle: before any barrier
+        if (tile * ET >= cnt) return;  // whole workgroup idle: before any barrier
         valid = env < cnt;
         env = valid ? env_list[env] : 0;
     }
-    for (int k = wave; k < a.npool; k += a.W)
-        As[k * 64 + lane] = valid ? nt_load2(Aplus + (int64_t)k * estride + env) : make_double2(0.0, 0.0);
+    {
+        const int k0 = HALF ? 2 * wave + (hi ? 1 : 0) : wave, kstep = HALF ? 2 * a.W : a.W;
+        for (int k = k0; k < a.npool; k += kstep)
+            As[k * ET + el] = valid ? nt_load2(Aplus + (int64_t)k * estride + env) : make_double2(0.0, 0.0);
+    }
     __syncthreads();
 
-    const char* lds = reinterpret_cast<const char*>(As) + 16 * lane;
-    double ep = 0.0;
+    const char* lds = reinterpret_cast<const char*>(As) + 16 * el;
+    double ep[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) ep[q] = 0.0;
     const int tb = wptr[part * a.W + wave], te = wptr[part * a.W + wave + 1];
     for (int ti = tb; ti < te; ++ti) {
         const int t = wlist[ti];
         const int slot = a.tasks[5 * t + 2], rb = a.tasks[5 * t + 3], re = a.tasks[5 * t + 4];
-        double2 acc = make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
+        double2 acc = (HALF && hi) ? make_double2(0.0, 0.0) : make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
         // P1, P2: prefix products of levels 1 and 2; Pc: the product of the
         // current record's whole prefix.  Runs are sorted, so every new run
         // recomputes its deepest level and Pc is always the last level computed
         // (1 for the prefix-free runs, which come first).
         double2 P1 = make_double2(1.0, 0.0), P2 = P1, Pc = P1;
-        const uint4* rp = reinterpret_cast<const uint4*>(a.recs) + 4 * (int64_t)rb;
-        uint4 h0 = rp[0], h1 = rp[1], h2 = rp[2], h3 = rp[3];
+        const uint4* rp = reinterpret_cast<const uint4*>(a.recs) + RQ * (int64_t)rb;
+        uint4 h0 = rp[0], h1 = rp[1], h2 = rp[2], h3 = rp[3], h4, h5;
+        if (CPLX) {
+            h4 = rp[4];
+            h5 = rp[5];
+        }
         // unrolled by 2: the prefetched and current record headers alternate
         // between two register sets instead of being copied every record
         // (measured 18% faster than the rolled loop)
 #pragma unroll 2
         for (int r = rb; r < re; ++r) {
             // next record (record nrec exists and is all zeros)
-            const uint4* np = reinterpret_cast<const uint4*>(a.recs) + 4 * (int64_t)(r + 1);
+            const uint4* np = reinterpret_cast<const uint4*>(a.recs) + RQ * (int64_t)(r + 1);
             const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            uint4 n4, n5;
+            if (CPLX) {
+                n4 = np[4];
+                n5 = np[5];
+            }
             const uint32_t ctl = h0.x;
-#define ACE_REC(T)                                                                                         \
-    case T:                                                                                                \
-        dagr_record<(T & 1) != 0, (T & 2) != 0, (T & 4) != 0, (T & 8) != 0>(lds, ctl, h0, h1, h2, h3, acc, \
-                                                                              P1, P2, Pc);                 \
+#define ACE_REC(T)                                                                                          \
+    case T:                                                                                                 \
+        dagr_record<(T & 1) != 0, (T & 2) != 0, (T & 4) != 0, (T & 8) != 0, CPLX, HALF>(                   \
+            lds, ctl, h0, h1, h2, h3, h4, h5, hi, acc, P1, P2, Pc);                                         \
         break;
             switch ((ctl >> 2) & 15) {
                 ACE_REC(0) ACE_REC(1) ACE_REC(2) ACE_REC(3) ACE_REC(4) ACE_REC(5) ACE_REC(6) ACE_REC(7)
@@ -298,34 +399,97 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
             h1 = n1;
             h2 = n2;
             h3 = n3;
+            if (CPLX) {
+                h4 = n4;
+                h5 = n5;
+            }
         }
-        if (slot < 0)
-            ep += acc.x;
-        else if (valid)
+        if (HALF) {
+            // the two halves' partial sums of the same environment
+            acc.x += __shfl_xor(acc.x, 32);
+            acc.y += __shfl_xor(acc.y, 32);
+        }
+        if (slot < 0) {
+            if (CPLX) {
+                const int c = -1 - slot;
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    if (c == q) {
+                        ep[2 * q] += acc.x;
+                        ep[2 * q + 1] += acc.y;
+                    }
+            } else {
+                ep[0] += acc.x;
+            }
+        } else if (valid && !hi) {
             nt_store2(Gbuf + (int64_t)slot * estride + env, acc);
+        }
     }
     __syncthreads();
-    double* red = reinterpret_cast<double*>(As);
-    red[wave * 64 + lane] = ep;
+    double* red = reinterpret_cast<double*>(As);  // [W][NE][ET]
+    if (!hi) {
+#pragma unroll
+        for (int q = 0; q < NE; ++q) red[(wave * NE + q) * ET + el] = ep[q];
+    }
     __syncthreads();
-    if (wave == 0 && valid && E_out) {
-        // one part: the energy; several: this part's share, summed by k_esum
-        double s = a.parts == 1 ? *e0p : 0.0;
-        for (int w = 0; w < a.W; ++w) s += red[w * 64 + lane];
-        E_out[(int64_t)part * estride + env] = s;
+    if (wave == 0 && !hi && valid && E_out) {
+        double Q[NE];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            Q[q] = (!CPLX && a.parts == 1) ? *e0p : 0.0;
+            for (int w = 0; w < a.W; ++w) Q[q] += red[(w * NE + q) * ET + el];
+        }
+        if (a.parts > 1) {
+            // this part's share, summed by k_esum
+#pragma unroll
+            for (int q = 0; q < NE; ++q) E_out[((int64_t)part * NE + q) * estride + env] = Q[q];
+        } else if (CPLX) {
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                double sum = e0p[p];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 u = a.cls_u[3 * c + p];
+                    sum += u.x * Q[2 * c] - u.y * Q[2 * c + 1];
+                }
+                E_out[env * 3 + p] = sum;
+            }
+        } else {
+            E_out[env] = Q[0];  // e0 + the waves' sums, in that order
+        }
     }
 }
 
 // E = e0 + sum of the parts' energy shares (fixed order: deterministic).
-// Environments from env_hi on belong to tiles with parts + 1 shares.
+// Environments from env_hi on belong to tiles with parts + 1 shares.  CPLX:
+// the shares are the class values Q_c, E_p = e0_p + Re sum_c u[c][p] Q_c.
+template <bool CPLX>
 __global__ void k_esum(int64_t nenv, int parts, int64_t env_hi, const double* __restrict__ Ep, int64_t estride,
-                       const double* __restrict__ e0p, double* __restrict__ E_out) {
+                       const double* __restrict__ e0p, const double2* __restrict__ cls_u, double* __restrict__ E_out) {
+    constexpr int NE = CPLX ? 6 : 1;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nenv) return;
-    double s = *e0p;
     const int np = e >= env_hi ? parts + 1 : parts;
-    for (int p = 0; p < np; ++p) s += Ep[(int64_t)p * estride + e];
-    E_out[e] = s;
+    double Q[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        Q[q] = CPLX ? 0.0 : *e0p;
+        for (int p = 0; p < np; ++p) Q[q] += Ep[((int64_t)p * NE + q) * estride + e];
+    }
+    if (CPLX) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            double sum = e0p[p];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double2 u = cls_u[3 * c + p];
+                sum += u.x * Q[2 * c] - u.y * Q[2 * c + 1];
+            }
+            E_out[e * 3 + p] = sum;
+        }
+    } else {
+        E_out[e] = Q[0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -341,7 +505,18 @@ static __device__ __align__(256) unsigned char scratch[ACE_SCRATCH_BYTES];
 static int upload_coeffs(ace_ctx* ctx, const double* c) {
     acehost::Coeffs cf = acehost::coefficients(ctx->host, c);
     if (ctx->leaf_coef.upload(cf.leaf.data(), cf.leaf.size())) return -1;
-    if (ctx->prop != ACE_PROP_INVARIANT) {
+    if (ctx->classed) {
+        // decoded leaves: {factor id (low word), 0, c.re, c.im}
+        const size_t nl = ctx->host.leaf_fid.size();
+        std::vector<double4> vl(std::max<size_t>(nl, 1));
+        for (size_t i = 0; i < nl; ++i) {
+            uint64_t bits = (uint32_t)ctx->host.leaf_fid[i];
+            double fid;
+            std::memcpy(&fid, &bits, sizeof(fid));
+            vl[i] = make_double4(fid, 0.0, cf.leaf[2 * i], cf.leaf[2 * i + 1]);
+        }
+        if (ctx->vleaf.upload(vl.data(), vl.size())) return -1;
+    } else if (ctx->prop != ACE_PROP_INVARIANT) {
         // decoded leaves: {factor id (low word), 0, c0, c1}, {c2, c3, c4, c5}
         const size_t nl = ctx->host.leaf_fid.size();
         std::vector<double4> vl(std::max<size_t>(2 * nl, 2));
@@ -355,9 +530,9 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
         }
         if (ctx->vleaf.upload(vl.data(), vl.size())) return -1;
     }
-    if (ctx->prop == ACE_PROP_INVARIANT) {
+    if (ctx->rec_et > 0) {
         std::vector<uint32_t> rec;
-        const std::string rerr = acehost::records(ctx->host, cf, rec);
+        const std::string rerr = acehost::records(ctx->host, cf, rec, ctx->rec_et == 64 ? 10 : 9);
         if (!rerr.empty()) return set_err(rerr);
         if (ctx->recs.upload(rec.data(), rec.size())) return -1;
     }
@@ -431,6 +606,13 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
     ctx->ngslots = pk->ngslots;
     ctx->nrec = pk->nrec;
     ctx->ncoef = pk->prop == ACE_PROP_INVARIANT ? 1 : 6;
+    ctx->classed = pk->classed;
+    // the record kernel's tile: 64 environments while their pooled values fit
+    // LDS (16 B each: npool <= 160), else 32 (npool <= 320)
+    if ((pk->prop == ACE_PROP_INVARIANT || pk->classed) && pk->nrec > 0)
+        ctx->rec_et = (size_t)pk->npool * 64 * sizeof(double2) <= 160 * 1024   ? 64
+                      : (size_t)pk->npool * 32 * sizeof(double2) <= 160 * 1024 ? 32
+                                                                               : 0;
     ctx->host = acehost::copy_pack(pk);
     const int nc = ctx->ncoef;
     auto fail = [&](const char* what) {
@@ -465,6 +647,13 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         ctx->gslot_ptr.upload(pk->gslot_ptr, pk->npool + 1) ||
         ctx->gslot_list.upload(pk->gslot_list, pk->gslot_ptr[pk->npool]))
         return fail("upload");
+    if (ctx->classed) {
+        double2 u[9];
+        for (int c = 0; c < 3; ++c)
+            for (int p = 0; p < 3; ++p) u[3 * c + p] = make_double2(ctx->host.cv.u[c][2 * p], ctx->host.cv.u[c][2 * p + 1]);
+        if (ctx->cls_u.upload(u, 9) || ctx->gslot_cls.upload(pk->gslot_class, std::max(1, pk->ngslots)))
+            return fail("upload");
+    }
     {
         // one record per A2B nonzero for the fused basis-value kernel: the AA
         // column's pooled factor ids and order, and (Invariant) the A2B value
@@ -548,7 +737,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             wc->d_ptr_e.upload(wc->ptr_e.data(), wc->ptr_e.size()) ||
             wc->d_list_e.upload(wc->list_e.data(), wc->list_e.size()))
             return fail("upload");
-        if (pk->prop == ACE_PROP_INVARIANT) {
+        if (ctx->rec_et > 0) {
             const std::vector<int> t_all = acehost::mode_tasks(ctx->host, true), t_e = acehost::mode_tasks(ctx->host, false);
             for (int P = 2; P <= MAX_PARTS; ++P) {
                 std::vector<int32_t> ptr, list;
@@ -683,6 +872,13 @@ extern "C" int ace_set_tile(ace_ctx* ctx, int W) {
     return set_err("ace_set_tile: waves per tile not in the pack");
 }
 
+extern "C" int ace_set_kernel(ace_ctx* ctx, int mode) {
+    if (!ctx) return set_err("ace_set_kernel: null context");
+    if (mode != 0 && mode != 1) return set_err("ace_set_kernel: mode must be 0 (automatic) or 1 (run interpreter)");
+    ctx->force_interp = mode == 1;
+    return 0;
+}
+
 extern "C" int ace_set_validation(ace_ctx* ctx, int on) {
     if (!ctx) return set_err("ace_set_validation: null context");
     ctx->validate = on != 0;
@@ -760,23 +956,23 @@ int validate_inputs(ace_ctx* ctx, int64_t nenv, const int32_t* off, int64_t nnbr
 template <int PROP, int ENVS, int MAXD>
 static int launch_dag_tile(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int64_t nenv, int64_t estride,
                            const Workspace& ws, double* E_out) {
-    constexpr int NC = PROP == 0 ? 1 : 3;
+    constexpr int NE = PROP == 0 ? 1 : PROP == 1 ? 3 : 6;  // energy partials per lane
     constexpr int SPW = 64 / ENVS;  // task streams per wave
     if (a.W % SPW) return set_err("tile configuration needs a multiple of 64 / ENVS task streams");
-    const size_t lds = std::max((size_t)ctx->npool * ENVS * sizeof(double2), (size_t)a.W * NC * ENVS * sizeof(double));
+    const size_t lds = std::max((size_t)ctx->npool * ENVS * sizeof(double2), (size_t)a.W * NE * ENVS * sizeof(double));
     const int64_t nt = (nenv + ENVS - 1) / ENVS;
     set_func_lds((const void*)k_dag<PROP, ENVS, MAXD>, lds);
     hipLaunchKernelGGL((k_dag<PROP, ENVS, MAXD>), dim3((unsigned)nt), dim3(64 * (a.W / SPW)), lds, st, a, nenv,
-                       ws.Aplus, estride, E_out, ws.Gbuf, ctx->e0v.p);
+                       ws.Aplus, estride, E_out, ws.Gbuf, ctx->e0v.p, ctx->cls_u.p);
     return 0;
 }
 
 template <int PROP, int MAXD>
 static int launch_dag_prop(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int64_t nenv, int64_t estride,
                            const Workspace& ws, double* E_out) {
-    constexpr int NC = PROP == 0 ? 1 : 3;
+    constexpr int NE = PROP == 0 ? 1 : PROP == 1 ? 3 : 6;
     auto fits = [&](int envs) {
-        return std::max((size_t)ctx->npool * envs * sizeof(double2), (size_t)a.W * NC * envs * sizeof(double)) <=
+        return std::max((size_t)ctx->npool * envs * sizeof(double2), (size_t)a.W * NE * envs * sizeof(double)) <=
                160 * 1024;
     };
     ctx->last_kernel = fits(64) ? 64 : fits(32) ? 32 : 16;
@@ -792,8 +988,27 @@ static int launch_dag_interp(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int
     if (ctx->prop == ACE_PROP_INVARIANT)
         return deep ? launch_dag_prop<0, 7>(ctx, st, a, nenv, estride, ws, E_out)
                     : launch_dag_prop<0, 3>(ctx, st, a, nenv, estride, ws, E_out);
+    if (ctx->classed)
+        return deep ? launch_dag_prop<2, 7>(ctx, st, a, nenv, estride, ws, E_out)
+                    : launch_dag_prop<2, 3>(ctx, st, a, nenv, estride, ws, E_out);
     return deep ? launch_dag_prop<1, 7>(ctx, st, a, nenv, estride, ws, E_out)
                 : launch_dag_prop<1, 3>(ctx, st, a, nenv, estride, ws, E_out);
+}
+
+// k_dagr for a tile of ET (64 or 32) environments.
+template <bool CPLX>
+static void launch_dagr(int ET, dim3 grid, dim3 block, size_t lds, hipStream_t st, const DagRecArgs& ra,
+                        int64_t nenv, const double2* Aplus, int64_t estride, double* E_out, double2* Gbuf,
+                        const double* e0p, const int32_t* env_list, const int32_t* env_count) {
+    if (ET == 64) {
+        set_func_lds((const void*)k_dagr<CPLX, false>, lds);
+        hipLaunchKernelGGL((k_dagr<CPLX, false>), grid, block, lds, st, ra, nenv, Aplus, estride, E_out, Gbuf, e0p,
+                           env_list, env_count);
+    } else {
+        set_func_lds((const void*)k_dagr<CPLX, true>, lds);
+        hipLaunchKernelGGL((k_dagr<CPLX, true>), grid, block, lds, st, ra, nenv, Aplus, estride, E_out, Gbuf, e0p,
+                           env_list, env_count);
+    }
 }
 
 // The launches of an energy (+ forces) evaluation, into a workspace that
@@ -801,9 +1016,7 @@ static int launch_dag_interp(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int
 // energy shares [estride] and, for the flat force kernel, nnbr map entries.
 static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
                      int64_t nnbr, const int8_t* species, double* E_out, double* dE, const Workspace& ws) {
-    const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
-    const int64_t ntile = (nenv + 63) / 64;
-    const int64_t estride = ntile * 64;
+    const int64_t estride = ((nenv + 63) / 64) * 64;
     const bool forces = dE != nullptr;
     WaveCfg* wc = ctx->wcfg[ctx->cur_w];
     // under stream capture (hipGraph) the timing events are left out of the graph
@@ -824,13 +1037,16 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
     a.wtask_list = forces ? wc->d_list_all.p : wc->d_list_e.p;
     a.npool = ctx->npool;
     a.W = wc->W;
-    size_t lds = (size_t)ctx->npool * 64 * sizeof(double2);
-    lds = std::max(lds, (size_t)wc->W * NC * 64 * sizeof(double));
-    // the record kernel: Invariant bases of order <= 4 whose 64-environment
-    // tile fits LDS; everything else runs the run interpreter
-    const bool use_records = ctx->prop == ACE_PROP_INVARIANT && ctx->nrec > 0 && lds <= 160 * 1024;
+    // the record kernel: Invariant and class-split EuclideanVector bases of
+    // order <= 4 with at most 320 pooled values; everything else runs the run
+    // interpreter
+    const bool use_records = ctx->rec_et > 0 && !ctx->force_interp;
     ctx->last_tail = 0;
     if (use_records) {
+        const int ET = ctx->rec_et;
+        const int NE = ctx->classed ? 6 : 1;
+        const int64_t ntile = (nenv + ET - 1) / ET;
+        const size_t lds = std::max((size_t)ctx->npool * ET * sizeof(double2), (size_t)wc->W * NE * ET * sizeof(double));
         // Parts per tile: the grid runs in rounds of ncu workgroups, and the last
         // round may be nearly empty; splitting each tile's tasks over P
         // workgroups shortens the rounds.  Pick P in 1..4 minimising
@@ -853,7 +1069,7 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
             }
         }
         ctx->last_parts = P;
-        ctx->last_kernel = 0;
+        ctx->last_kernel = ET == 64 ? 0 : 1;
         // tail tiles with P + 1 parts (P >= 2, so every tile sums shares in k_esum)
         int64_t nhi = 0;
         if (ctx->tail >= 0) {
@@ -876,6 +1092,7 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
         ra.task_const = ctx->task_const.p;
         ra.wtask_ptr = P == 1 ? a.wtask_ptr : (forces ? wc->pp_all[P].p : wc->pp_e[P].p);
         ra.wtask_list = P == 1 ? a.wtask_list : (forces ? wc->pl_all[P].p : wc->pl_e[P].p);
+        ra.cls_u = ctx->cls_u.p;
         ra.npool = ctx->npool;
         ra.W = wc->W;
         ra.parts = P;
@@ -883,13 +1100,22 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
         ra.wtask_ptr_hi = nhi > 0 ? (forces ? wc->pp_all[P + 1].p : wc->pp_e[P + 1].p) : nullptr;
         ra.wtask_list_hi = nhi > 0 ? (forces ? wc->pl_all[P + 1].p : wc->pl_e[P + 1].p) : nullptr;
         const int64_t nblk = (ntile - nhi) * P + nhi * (P + 1);
-        set_func_lds((const void*)k_dagr, lds);
-        hipLaunchKernelGGL(k_dagr, dim3((unsigned)nblk), dim3(64 * wc->W), lds, st, ra, nenv,
-                           ws.Aplus, estride, P == 1 ? E_out : ws.Epart, ws.Gbuf, ctx->e0v.p,
-                           (const int32_t*)nullptr, (const int32_t*)nullptr);
-        if (P > 1)
-            hipLaunchKernelGGL(k_esum, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, P,
-                               (ntile - nhi) * 64, ws.Epart, estride, ctx->e0v.p, E_out);
+        double* Eo = P == 1 ? E_out : ws.Epart;
+        if (ctx->classed)
+            launch_dagr<true>(ET, dim3((unsigned)nblk), dim3(64 * wc->W), lds, st, ra, nenv, ws.Aplus, estride, Eo,
+                              ws.Gbuf, ctx->e0v.p, nullptr, nullptr);
+        else
+            launch_dagr<false>(ET, dim3((unsigned)nblk), dim3(64 * wc->W), lds, st, ra, nenv, ws.Aplus, estride, Eo,
+                               ws.Gbuf, ctx->e0v.p, nullptr, nullptr);
+        if (P > 1) {
+            const dim3 g((unsigned)((nenv + 255) / 256)), b(256);
+            if (ctx->classed)
+                hipLaunchKernelGGL(k_esum<true>, g, b, 0, st, nenv, P, (ntile - nhi) * ET, ws.Epart, estride,
+                                   ctx->e0v.p, ctx->cls_u.p, E_out);
+            else
+                hipLaunchKernelGGL(k_esum<false>, g, b, 0, st, nenv, P, (ntile - nhi) * ET, ws.Epart, estride,
+                                   ctx->e0v.p, ctx->cls_u.p, E_out);
+        }
     } else {
         ctx->last_parts = 1;
         if (launch_dag_interp(ctx, st, a, nenv, estride, ws, E_out)) return -1;
@@ -909,12 +1135,11 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     if (validate_inputs(ctx, nenv, off, nnbr, species, (hipStream_t)stream)) return -1;
     if (nenv == 0) return 0;
     if (!E_out) return set_err("null E_out");
-    const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
     const int64_t estride = ((nenv + 63) / 64) * 64;
     const bool forces = dE != nullptr;
     if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
-    if (forces && ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * NC * estride)) return -1;
-    if (ctx->prop == ACE_PROP_INVARIANT && ctx->Epart.ensure((size_t)MAX_PARTS * estride)) return -1;
+    if (forces && ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * gslot_width(ctx) * estride)) return -1;
+    if (ctx->rec_et > 0 && ctx->Epart.ensure((size_t)MAX_PARTS * (ctx->classed ? 6 : 1) * estride)) return -1;
     if (forces && ctx->flat_force && nnbr > 0 && ctx->nbr_env.ensure(nnbr)) return -1;
     return eval_path(ctx, (hipStream_t)stream, nenv, off, xyz, nnbr, species, E_out, dE, ctx_buffers(ctx));
 }
@@ -931,12 +1156,11 @@ extern "C" int ace_energy_forces_noalloc(ace_ctx* ctx, int64_t nenv, const int32
     if (!ctx->scratch) return set_err("ace_energy_forces_noalloc: no scratch array on this device");
     HIP_TRY(hipSetDevice(ctx->device));
     // carve the fixed array: A+, G, the energy shares, the neighbour map
-    const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
     const int64_t estride = ((nenv + 63) / 64) * 64;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t bA = up((size_t)ctx->npool * estride * sizeof(double2));
-    const size_t bG = up((size_t)std::max(1, ctx->ngslots) * NC * estride * sizeof(double2));
-    const size_t bE = ctx->prop == ACE_PROP_INVARIANT ? up((size_t)MAX_PARTS * estride * sizeof(double)) : 0;
+    const size_t bG = up((size_t)std::max(1, ctx->ngslots) * gslot_width(ctx) * estride * sizeof(double2));
+    const size_t bE = ctx->rec_et > 0 ? up((size_t)MAX_PARTS * (ctx->classed ? 6 : 1) * estride * sizeof(double)) : 0;
     const size_t bN = ctx->flat_force ? up((size_t)nnbr * sizeof(int32_t)) : 0;
     const size_t need = bA + bG + bE + bN;
     if (need > ACE_SCRATCH_BYTES)
@@ -982,6 +1206,7 @@ int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estrid
                       const int32_t* env_list, const int32_t* env_count) {
     if (ctx->prop != ACE_PROP_INVARIANT || ctx->nrec == 0)
         return set_err("listed product-DAG evaluation: Invariant bases of correlation order <= 4 only");
+    if (ctx->rec_et != 64) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     WaveCfg* wc = ctx->wcfg[ctx->cur_w];
     const int64_t ntile = (nenv + 63) / 64;
     DagRecArgs ra;
@@ -990,6 +1215,7 @@ int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estrid
     ra.task_const = ctx->task_const.p;
     ra.wtask_ptr = forces ? wc->d_ptr_all.p : wc->d_ptr_e.p;
     ra.wtask_list = forces ? wc->d_list_all.p : wc->d_list_e.p;
+    ra.cls_u = nullptr;
     ra.npool = ctx->npool;
     ra.W = wc->W;
     ra.parts = 1;
@@ -997,10 +1223,8 @@ int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estrid
     ra.wtask_ptr_hi = nullptr;
     ra.wtask_list_hi = nullptr;
     const size_t lds = std::max((size_t)ctx->npool * 64 * sizeof(double2), (size_t)wc->W * 64 * sizeof(double));
-    if (lds > 160 * 1024) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
-    set_func_lds((const void*)k_dagr, lds);
-    hipLaunchKernelGGL(k_dagr, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, ra, nenv, ctx->Aplus.p, estride,
-                       E_out, forces ? ctx->Gbuf.p : nullptr, ctx->e0v.p, env_list, env_count);
+    launch_dagr<false>(64, dim3((unsigned)ntile), dim3(64 * wc->W), lds, st, ra, nenv, ctx->Aplus.p, estride, E_out,
+                       forces ? ctx->Gbuf.p : nullptr, ctx->e0v.p, env_list, env_count);
     HIP_TRY(hipGetLastError());
     return 0;
 }

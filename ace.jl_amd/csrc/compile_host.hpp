@@ -86,7 +86,8 @@ struct PackStore {
     std::vector<double> a2b_vals;
     std::vector<int32_t> runs, leaf_fid, leaf_term_ptr, tasks, task_term_ptr, term_row, term_conj;
     std::vector<double> term_scale;
-    std::vector<int32_t> gslot_ptr, gslot_list;
+    std::vector<int32_t> gslot_ptr, gslot_list, gslot_class;
+    int classed = 0;
     std::vector<uint32_t> rec_hdr;
     std::vector<int32_t> rec_leaf;
     std::vector<int32_t> feat_rows, feat_ptr, feat_f, fgrad_ptr, fgrad_feat, fgrad_f;
@@ -140,6 +141,8 @@ struct PackStore {
         p.ngslots = ngslots;
         p.gslot_ptr = gslot_ptr.data();
         p.gslot_list = gslot_list.data();
+        p.classed = classed;
+        p.gslot_class = gslot_class.data();
         p.nrec = (int32_t)(rec_hdr.size() / 8);
         p.rec_hdr = rec_hdr.data();
         p.rec_leaf = rec_leaf.data();
@@ -335,6 +338,19 @@ struct Program {
     }
 };
 
+inline int term_class(const Term& t, const std::vector<int8_t>& row_class) {
+    const int c = row_class[(size_t)t.row];
+    return t.conj ? 2 - c : c;
+}
+
+// The polynomial as three polynomials, one per term class.
+inline std::vector<Poly> split_by_class(const Poly& poly, const std::vector<int8_t>& row_class) {
+    std::vector<Poly> out(3);
+    for (size_t i = 0; i < poly.size(); ++i)
+        for (const Term& t : poly.terms[i]) out[(size_t)term_class(t, row_class)].add(poly.keys[i], t);
+    return out;
+}
+
 // Longest-first assignment of tasks to W waves (ties to the lowest wave);
 // every list ascending.
 inline std::vector<std::vector<int>> lpt_assign(const std::vector<double>& costs, int W) {
@@ -497,32 +513,62 @@ inline std::string compile_jl(const ace_jl_basis* b, PackStore& ps) {
             }
         }
     }
-    // 5. tasks: the energy in ENERGY_PARTS parts, then each G[k] (split when long)
+    // 5. tasks: the energy in ENERGY_PARTS parts, then each G[k] (split when
+    // long).  EuclideanVector maps that allow it are split by class: the
+    // energy into one task group per class polynomial Q_M (output slot
+    // -1 - class), each G[k] into its class parts (slot class in gslot_class).
+    const ace_pack v0 = ps.view();
+    const std::vector<int8_t> row_class = pack_row_classes(&v0);
+    ps.classed = (b->prop == ACE_PROP_EUCLIDEAN_VECTOR &&
+                  class_vectors(ps.a2b_cols.data(), ps.a2b_vals.data(), ps.a2b_cols.size(), row_class.data()).ok)
+                     ? 1
+                     : 0;
     Program prog;
     std::vector<int> energy_tasks;
-    for (const Poly& part : split_poly(merged, ENERGY_PARTS)) {
-        int t = 0;
-        err = prog.add_task(part, -1, t);
-        if (!err.empty()) return err;
-        energy_tasks.push_back(t);
+    {
+        std::vector<std::pair<int, Poly>> groups;
+        if (ps.classed) {
+            std::vector<Poly> q = split_by_class(merged, row_class);
+            for (int c = 0; c < 3; ++c)
+                if (!q[(size_t)c].empty()) groups.emplace_back(-1 - c, std::move(q[(size_t)c]));
+        } else {
+            groups.emplace_back(-1, merged);
+        }
+        for (auto& gr : groups)
+            for (const Poly& part : split_poly(gr.second, ENERGY_PARTS)) {
+                int t = 0;
+                err = prog.add_task(part, gr.first, t);
+                if (!err.empty()) return err;
+                energy_tasks.push_back(t);
+            }
     }
     std::vector<std::vector<int32_t>> gslots((size_t)npool);
     int nslot = 0;
     for (int k = 0; k < npool; ++k) {
         const Poly& g = G[(size_t)k];
         if (g.empty()) continue;
-        const int nparts = std::max(1, (int)std::ceil((double)g.size() / MAX_TASK_COST));
-        std::vector<Poly> parts;
-        if (nparts > 1)
-            parts = split_poly(g, nparts);
+        std::vector<Poly> byc;
+        if (ps.classed)
+            byc = split_by_class(g, row_class);
         else
-            parts.push_back(g);
-        for (const Poly& part : parts) {
-            int t = 0;
-            err = prog.add_task(part, nslot, t);
-            if (!err.empty()) return err;
-            gslots[(size_t)k].push_back(nslot);
-            ++nslot;
+            byc.push_back(g);
+        for (size_t c = 0; c < byc.size(); ++c) {
+            const Poly& gc = byc[c];
+            if (gc.empty()) continue;
+            const int nparts = std::max(1, (int)std::ceil((double)gc.size() / MAX_TASK_COST));
+            std::vector<Poly> parts;
+            if (nparts > 1)
+                parts = split_poly(gc, nparts);
+            else
+                parts.push_back(gc);
+            for (const Poly& part : parts) {
+                int t = 0;
+                err = prog.add_task(part, nslot, t);
+                if (!err.empty()) return err;
+                gslots[(size_t)k].push_back(nslot);
+                if (ps.classed) ps.gslot_class.push_back((int32_t)c);
+                ++nslot;
+            }
         }
     }
     ps.ngslots = nslot;
@@ -534,9 +580,10 @@ inline std::string compile_jl(const ace_jl_basis* b, PackStore& ps) {
     }
     ps.runs = prog.runs;
     ps.leaf_fid = prog.leaf_fid;
-    // 6. records (Invariant bases of order <= 4 run the record kernel)
+    // 6. records (Invariant and class-split EuclideanVector bases of order <= 4
+    // run the record kernel)
     const int ntasks = (int)prog.task_terms.size();
-    const bool records = mo <= 4;
+    const bool records = mo <= 4 && (b->prop == ACE_PROP_INVARIANT || ps.classed);
     for (int t = 0; t < ntasks; ++t) {
         const int32_t rb = (int32_t)(ps.rec_hdr.size() / 8);
         for (int r = prog.tasks[3 * (size_t)t]; records && r < prog.tasks[3 * (size_t)t + 1]; ++r) {

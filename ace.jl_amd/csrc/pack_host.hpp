@@ -6,6 +6,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -19,6 +20,101 @@ struct WaveTable {
     int W;
     std::vector<int32_t> ptr_all, list_all, ptr_e, list_e;
 };
+
+// Class c = M + 1 of every AA row of a pack, M = the total m of its factors
+// (pooled m, negated for conjugated factors); -1 for the order-0 row.
+inline std::vector<int8_t> pack_row_classes(const ace_pack* pk) {
+    std::vector<int8_t> out((size_t)pk->nAA, (int8_t)-1);
+    for (int i = 0; i < pk->nAA; ++i) {
+        const int o = pk->orders[i];
+        if (!o) continue;
+        int M = 0;
+        for (int t = 0; t < o; ++t) {
+            const int f = pk->aspec_fid[pk->iAA2iA[(size_t)i * pk->maxorder + t]];
+            const int m = pk->pool[4 * (f & 0x7fff) + 3];
+            M += (f & 0x8000) ? -m : m;
+        }
+        out[(size_t)i] = (int8_t)std::max(-2, std::min(M + 1, 3));  // outside [0, 2]: not a vector class
+    }
+    return out;
+}
+
+// The class structure of a EuclideanVector A2B map (symmbasis.jl:267-285 with
+// the l = 1 coupling of the vector property): every value of a column of
+// class c should be one complex number times a unit vector u_c, and u_c, u_{2-c}
+// should be conjugate up to a phase (Y_1^{-M} = (-1)^M conj(Y_1^M)).
+//   u[c]    the value of largest norm among the class's entries (the first
+//           such), normalised; a class without entries takes conj(u) of its
+//           mirror (or stays zero)
+//   sig[c]  <conj(u_c), u_{2-c}> = sum_p conj(u_c,p) conj(u_{2-c},p), so that
+//           conj(u_c) = sig[c] u_{2-c}
+//   ok      every entry lies within tol * (largest norm of the map) of its
+//           class line, and |sig[c]| >= 1 - tol wherever both classes occur
+struct ClassVectors {
+    double u[3][6] = {};
+    double sig[3][2] = {};
+    bool ok = false;
+};
+
+inline ClassVectors class_vectors(const int32_t* cols, const double* vals, size_t nnz, const int8_t* row_class,
+                                  double tol = 1e-12) {
+    ClassVectors cv;
+    auto norm = [](const double* v) {
+        double n2 = 0.0;
+        for (int q = 0; q < 6; ++q) n2 += v[q] * v[q];
+        return std::sqrt(n2);
+    };
+    double best[3] = {0.0, 0.0, 0.0}, maxn = 0.0;
+    long ref[3] = {-1, -1, -1};
+    for (size_t z = 0; z < nnz; ++z) {
+        const int c = row_class[cols[z]];
+        if (c < 0 || c > 2) return cv;
+        const double n = norm(vals + 6 * z);
+        maxn = std::max(maxn, n);
+        if (n > best[c]) {
+            best[c] = n;
+            ref[c] = (long)z;
+        }
+    }
+    for (int c = 0; c < 3; ++c)
+        if (ref[c] >= 0)
+            for (int q = 0; q < 6; ++q) cv.u[c][q] = vals[6 * ref[c] + q] / best[c];
+    for (int c = 0; c < 3; ++c)
+        if (ref[c] < 0 && ref[2 - c] >= 0)
+            for (int p = 0; p < 3; ++p) {
+                cv.u[c][2 * p] = cv.u[2 - c][2 * p];
+                cv.u[c][2 * p + 1] = -cv.u[2 - c][2 * p + 1];
+            }
+    for (size_t z = 0; z < nnz; ++z) {
+        const int c = row_class[cols[z]];
+        const double* v = vals + 6 * z;
+        const double* u = cv.u[c];
+        double dr = 0.0, di = 0.0;  // <v, u> = sum_p v_p conj(u_p)
+        for (int p = 0; p < 3; ++p) {
+            dr += v[2 * p] * u[2 * p] + v[2 * p + 1] * u[2 * p + 1];
+            di += v[2 * p + 1] * u[2 * p] - v[2 * p] * u[2 * p + 1];
+        }
+        double r[6];
+        for (int p = 0; p < 3; ++p) {
+            r[2 * p] = v[2 * p] - (dr * u[2 * p] - di * u[2 * p + 1]);
+            r[2 * p + 1] = v[2 * p + 1] - (dr * u[2 * p + 1] + di * u[2 * p]);
+        }
+        if (norm(r) > tol * maxn) return cv;
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double *a = cv.u[c], *b = cv.u[2 - c];
+        double sr = 0.0, si = 0.0;  // sum_p conj(a_p) conj(b_p) = conj(sum_p a_p b_p)
+        for (int p = 0; p < 3; ++p) {
+            sr += a[2 * p] * b[2 * p] - a[2 * p + 1] * b[2 * p + 1];
+            si += a[2 * p] * b[2 * p + 1] + a[2 * p + 1] * b[2 * p];
+        }
+        cv.sig[c][0] = sr;
+        cv.sig[c][1] = -si;
+        if (ref[c] >= 0 && ref[2 - c] >= 0 && std::sqrt(sr * sr + si * si) < 1.0 - tol) return cv;
+    }
+    cv.ok = true;
+    return cv;
+}
 
 // Returns "" when the pack is consistent, else a message.
 inline std::string validate_pack(const ace_pack* pk) {
@@ -89,7 +185,8 @@ inline std::string validate_pack(const ace_pack* pk) {
     }
     for (int t = 0; t < pk->ntasks; ++t) {
         const int32_t* tk = pk->tasks + 5 * (size_t)t;
-        if (tk[0] < 0 || tk[1] < tk[0] || tk[1] > pk->nruns || tk[2] < -1 || tk[2] >= pk->ngslots)
+        if (tk[0] < 0 || tk[1] < tk[0] || tk[1] > pk->nruns || tk[2] < (pk->classed ? -3 : -1) ||
+            tk[2] >= pk->ngslots)
             return "task out of range";
         if (tk[3] < 0 || tk[4] < tk[3] || tk[4] > pk->nrec) return "task out of range";
         if (pk->task_term_ptr[t] < 0 || pk->task_term_ptr[t + 1] < pk->task_term_ptr[t] ||
@@ -104,6 +201,16 @@ inline std::string validate_pack(const ace_pack* pk) {
     if (pk->gslot_ptr[pk->npool] > pk->ngslots) return "bad gslot table";
     for (int s = 0; s < pk->gslot_ptr[pk->npool]; ++s)
         if (pk->gslot_list[s] < 0 || pk->gslot_list[s] >= pk->ngslots) return "gslot out of range";
+    if (pk->classed != 0 && pk->classed != 1) return "classed must be 0 or 1";
+    if (pk->classed) {
+        if (pk->prop != ACE_PROP_EUCLIDEAN_VECTOR) return "class-split programs need a EuclideanVector basis";
+        if (pk->ngslots > 0 && !pk->gslot_class) return "null gslot_class";
+        for (int s = 0; s < pk->ngslots; ++s)
+            if (pk->gslot_class[s] < 0 || pk->gslot_class[s] > 2) return "gslot class out of range";
+        const std::vector<int8_t> rc = pack_row_classes(pk);
+        if (!class_vectors(pk->a2b_cols, pk->a2b_vals, (size_t)pk->a2b_nnz, rc.data()).ok)
+            return "classed pack, but the A2B map is not one complex number times a class vector per column";
+    }
     if (pk->nfeat < 0 || pk->nfeat > 160) return "nfeat must be in [0, 160]";
     if (pk->nfeat > 0) {
         if (!pk->feat_rows || !pk->feat_ptr || !pk->feat_f || !pk->feat_coef || !pk->fgrad_ptr)
@@ -177,7 +284,13 @@ struct PackCopy {
     std::vector<int32_t> tasks;  // [ntasks][5] (run_b, run_e, slot, rec_b, rec_e)
     std::vector<int32_t> rowptr, cols, leaf_term_ptr, task_term_ptr, term_row, term_conj, orders;
     std::vector<double> vals, term_scale;
+    // class-split EuclideanVector programs: row classes and class vectors
+    int classed = 0;
+    std::vector<int8_t> row_class;
+    ClassVectors cv;
     int ncoef() const { return prop == ACE_PROP_INVARIANT ? 1 : 6; }
+    // doubles per leaf / task coefficient: 1 real, 2 (one complex, class-split), 6
+    int lcoef() const { return prop == ACE_PROP_INVARIANT ? 1 : classed ? 2 : 6; }
 };
 
 inline PackCopy copy_pack(const ace_pack* pk) {
@@ -202,6 +315,11 @@ inline PackCopy copy_pack(const ace_pack* pk) {
     p.rec_hdr.assign(pk->rec_hdr, pk->rec_hdr + 8 * (size_t)pk->nrec);
     p.rec_leaf.assign(pk->rec_leaf, pk->rec_leaf + 4 * (size_t)pk->nrec);
     p.tasks.assign(pk->tasks, pk->tasks + 5 * (size_t)pk->ntasks);
+    p.classed = pk->classed;
+    if (p.classed) {
+        p.row_class = pack_row_classes(pk);
+        p.cv = class_vectors(pk->a2b_cols, pk->a2b_vals, (size_t)pk->a2b_nnz, p.row_class.data());
+    }
     return p;
 }
 
@@ -223,6 +341,46 @@ inline Coeffs coefficients(const PackCopy& p, const double* c) {
         }
     };
     Coeffs out;
+    for (int r = 0; r < p.nAA; ++r)
+        if (p.orders[r] == 0)
+            for (int q = 0; q < nc; ++q) out.e0[q] += ct[(size_t)r * nc + q];
+    if (p.classed) {
+        // c~[row] = s[row] u_M with s[row] = <c~[row], u_M>; a term's scalar is
+        // scale s[row], or scale sig_M conj(s[row]) when conjugated (class -M)
+        std::vector<double> sc((size_t)p.nAA * 2, 0.0);
+        for (int r = 0; r < p.nAA; ++r) {
+            const int c = p.row_class[(size_t)r];
+            if (c < 0) continue;
+            const double* v = &ct[(size_t)r * 6];
+            const double* u = p.cv.u[c];
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                sr += v[2 * q] * u[2 * q] + v[2 * q + 1] * u[2 * q + 1];
+                si += v[2 * q + 1] * u[2 * q] - v[2 * q] * u[2 * q + 1];
+            }
+            sc[2 * (size_t)r] = sr;
+            sc[2 * (size_t)r + 1] = si;
+        }
+        auto add_scalar = [&](int tt, double* o) {
+            const int row = p.term_row[tt];
+            const double s = p.term_scale[tt], sr = sc[2 * (size_t)row], si = sc[2 * (size_t)row + 1];
+            if (p.term_conj[tt]) {
+                const double* g = p.cv.sig[p.row_class[(size_t)row]];
+                o[0] += s * (g[0] * sr + g[1] * si);  // sig conj(s)
+                o[1] += s * (g[1] * sr - g[0] * si);
+            } else {
+                o[0] += s * sr;
+                o[1] += s * si;
+            }
+        };
+        out.leaf.assign((size_t)p.nleaves * 2, 0.0);
+        for (int i = 0; i < p.nleaves; ++i)
+            for (int tt = p.leaf_term_ptr[i]; tt < p.leaf_term_ptr[i + 1]; ++tt) add_scalar(tt, &out.leaf[2 * (size_t)i]);
+        out.task.assign((size_t)p.ntasks * 2, 0.0);
+        for (int t = 0; t < p.ntasks; ++t)
+            for (int tt = p.task_term_ptr[t]; tt < p.task_term_ptr[t + 1]; ++tt) add_scalar(tt, &out.task[2 * (size_t)t]);
+        return out;
+    }
     out.leaf.assign((size_t)p.nleaves * nc, 0.0);
     for (int i = 0; i < p.nleaves; ++i)
         for (int tt = p.leaf_term_ptr[i]; tt < p.leaf_term_ptr[i + 1]; ++tt) add_term(tt, &out.leaf[(size_t)i * nc]);
@@ -233,31 +391,36 @@ inline Coeffs coefficients(const PackCopy& p, const double* c) {
         for (int tt = p.task_term_ptr[t]; tt < p.task_term_ptr[t + 1]; ++tt) add_term(tt, tmp);
         for (int q = 0; q < nc; ++q) out.task[(size_t)t * tcn + q] = tmp[q];
     }
-    for (int r = 0; r < p.nAA; ++r)
-        if (p.orders[r] == 0)
-            for (int q = 0; q < nc; ++q) out.e0[q] += ct[(size_t)r * nc + q];
     return out;
 }
 
-// Device records (Invariant), decoded for the kernel: 64 bytes each.
+// Device records, decoded for the kernel: 64 bytes each (real leaf
+// coefficients, Invariant), 96 bytes (one complex coefficient per leaf,
+// class-split EuclideanVector programs).
 //   dw0    control: REC_MORE (leaves 2 and 3 present), REC_U1 / REC_U2 /
 //          REC_U3 (prefix level 1 / 2 / 3 is recomputed) -- together the
 //          record type in bits 2-5 --, REC_CONJ (the leaves are conjugated:
 //          one flag per record, since the leaf's m is fixed by the prefix)
-//   dw1-4  leaf i: LDS byte offset of its pooled value (k << 10; 64 lanes x 16 B)
+//   dw1-4  leaf i: LDS byte offset of its pooled value (k << lds_shift: 64
+//          lanes x 16 B at shift 10, 32 lanes at shift 9)
 //   dw5-7  prefix factor t: LDS byte offset | conjugation in bit 31
-//   dw8-15 leaf coefficients (FP64; zero for absent leaves)
+//   dw8-15 leaf coefficients (FP64; zero for absent leaves), or
+//   dw8-23 complex leaf coefficients (re, im), stored conjugated when the
+//          leaves are: sum c conj(A) = conj(sum conj(c) A)
 // One zero record follows the last so the kernel can always load record r + 1.
 constexpr uint32_t REC_MORE = 1u << 2, REC_U1 = 1u << 3, REC_U2 = 1u << 4, REC_U3 = 1u << 5;
 constexpr uint32_t REC_CONJ = 1u << 31;
 
+inline int rec_words(const PackCopy& p) { return p.classed ? 24 : 16; }
+
 // Returns "" or a message (a record whose leaves mix conjugation).
-inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint32_t>& out) {
-    out.assign(16 * ((size_t)p.nrec + 1), 0u);
-    auto off = [](uint32_t f) { return (f & 0x7fffu) << 10; };
+inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint32_t>& out, int lds_shift = 10) {
+    const int RW = rec_words(p);
+    out.assign((size_t)RW * ((size_t)p.nrec + 1), 0u);
+    auto off = [&](uint32_t f) { return (f & 0x7fffu) << lds_shift; };
     for (int r = 0; r < p.nrec; ++r) {
         const uint32_t* h = p.rec_hdr.data() + 8 * (size_t)r;
-        uint32_t* o = out.data() + 16 * (size_t)r;
+        uint32_t* o = out.data() + (size_t)RW * r;
         const int d = h[0] & 3, j = (h[0] >> 2) & 7, nl = (h[0] >> 5) & 7;
         uint32_t ctl = 0;
         if (nl > 2) ctl |= REC_MORE;
@@ -273,12 +436,23 @@ inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint
         o[0] = ctl;
         for (int i = 0; i < 4; ++i) o[1 + i] = i < nl ? off(lid[i]) : 0u;
         for (int t = 0; t < 3; ++t) o[5 + t] = off(fid[t]) | ((fid[t] & 0x8000u) << 16);
-        double c[4] = {0, 0, 0, 0};
-        for (int i = 0; i < 4; ++i) {
-            const int lf = p.rec_leaf[4 * (size_t)r + i];
-            if (lf >= 0) c[i] = cf.leaf[lf];
+        if (p.classed) {
+            double c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < 4; ++i) {
+                const int lf = p.rec_leaf[4 * (size_t)r + i];
+                if (lf < 0) continue;
+                c[2 * i] = cf.leaf[2 * (size_t)lf];
+                c[2 * i + 1] = cj ? -cf.leaf[2 * (size_t)lf + 1] : cf.leaf[2 * (size_t)lf + 1];
+            }
+            std::memcpy(o + 8, c, sizeof(c));
+        } else {
+            double c[4] = {0, 0, 0, 0};
+            for (int i = 0; i < 4; ++i) {
+                const int lf = p.rec_leaf[4 * (size_t)r + i];
+                if (lf >= 0) c[i] = cf.leaf[lf];
+            }
+            std::memcpy(o + 8, c, sizeof(c));
         }
-        std::memcpy(o + 8, c, sizeof(c));
     }
     return "";
 }

@@ -99,7 +99,7 @@ extern "C" int64_t acehost_compiled_array(void* p, const char* name, void* out) 
     ACE_ARR(rA) ACE_ARR(rB) ACE_ARR(rC) ACE_ARR(pool) ACE_ARR(aspec_fid) ACE_ARR(aspec_sgn) ACE_ARR(orders)
     ACE_ARR(iAA2iA) ACE_ARR(a2b_rowptr) ACE_ARR(a2b_cols) ACE_ARR(a2b_vals) ACE_ARR(runs) ACE_ARR(leaf_fid)
     ACE_ARR(leaf_term_ptr) ACE_ARR(tasks) ACE_ARR(task_term_ptr) ACE_ARR(term_row) ACE_ARR(term_scale)
-    ACE_ARR(term_conj) ACE_ARR(gslot_ptr) ACE_ARR(gslot_list) ACE_ARR(rec_hdr) ACE_ARR(rec_leaf) ACE_ARR(feat_rows)
+    ACE_ARR(term_conj) ACE_ARR(gslot_ptr) ACE_ARR(gslot_list) ACE_ARR(gslot_class) ACE_ARR(rec_hdr) ACE_ARR(rec_leaf) ACE_ARR(feat_rows)
     ACE_ARR(feat_ptr) ACE_ARR(feat_f) ACE_ARR(feat_coef) ACE_ARR(fgrad_ptr) ACE_ARR(fgrad_feat) ACE_ARR(fgrad_f)
     ACE_ARR(fgrad_coef) ACE_ARR(wave_cfg_W) ACE_ARR(wave_task_ptr) ACE_ARR(wave_task_list) ACE_ARR(wave_etask_ptr)
     ACE_ARR(wave_etask_list)

@@ -60,10 +60,21 @@ def algorithmic_flops(pack, forces=True):
 
 def executed_flops(pack, forces=True):
     """FLOPs the product-DAG kernel executes per environment: 4 per leaf
-    (real x complex accumulate), 6 per prefix product, 8 per run (complex
+    (real x complex accumulate; 8 for the complex coefficients of class-split
+    EuclideanVector programs), 6 per prefix product, 8 per run (complex
     multiply-accumulate of prefix x leaf sum)."""
     w = pack.work_counts(forces)
-    return 4.0 * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
+    per_leaf = 8.0 if getattr(pack, "classed", 0) else 4.0
+    return per_leaf * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
+
+
+def dag_kernel_name(code):
+    """The product-DAG kernel of ace_last_dag_shape's code."""
+    if code == 0:
+        return "k_dagr (64-environment tiles)"
+    if code == 1:
+        return "k_dagr (32-environment tiles, two half-waves per record)"
+    return f"k_dag run interpreter ({code}-environment tiles)"
 
 
 def cpu_model() -> str:
@@ -347,7 +358,7 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": DAG_TRAFFIC_BYTES.get(b.nenv) if headline else None,
                      "traffic_source": TRAFFIC_SOURCE if headline and b.nenv in DAG_TRAFFIC_BYTES else None,
-                     "kernel": "k_dagr" if dag_kernel == 0 else f"k_dag ({dag_kernel}-environment tiles)",
+                     "kernel": dag_kernel_name(dag_kernel),
                      "kernel_ms": round(dag_avg, 4), "dag_parts": dag_parts, "dag_tail_tiles": dag_tail,
                      "flops_per_env_algorithmic": f_alg, "flops_per_env_executed": f_exe,
                      "executed_tflops": round(achieved_exe, 3),

@@ -107,6 +107,13 @@ typedef struct ace_pack {
     int32_t ngslots;              /* number of G output slots */
     const int32_t* gslot_ptr;     /* [npool+1] slots summed into G[k] */
     const int32_t* gslot_list;    /* [ngslots] */
+    /* EuclideanVector programs split by class (1) or by component (0): with
+     * classed = 1 every AA column's A2B values are one complex number times
+     * a fixed unit vector u_M of its class M = total m (in {-1, 0, 1}), the
+     * energy tasks of class c = M + 1 have output slot -1 - c, and
+     * gslot_class[slot] is the class of a G slot. */
+    int32_t classed;
+    const int32_t* gslot_class;   /* [ngslots] when classed */
     /* the same programs as 64-byte records (Invariant, correlation order <= 4;
      * nrec = 0 otherwise): 8 header dwords (dw0 = depth | jstart << 2 |
      * nleaf << 5, dw1 = f1 | f2 << 16, dw2 = f3, dw3.. leaf factor ids two per
@@ -216,7 +223,7 @@ int ace_set_tile(ace_ctx* ctx, int waves_per_tile);
  * 0 (default) chooses 1 to 4 per call from the tile and CU counts. */
 int ace_set_parts(ace_ctx* ctx, int parts);
 
-/* Tiles at the end of the product-DAG grid (Invariant) that run one more
+/* Tiles at the end of the product-DAG grid (record kernel) that run one more
  * part than the rest, so that the last round holds shorter workgroups.
  * -1 (default) chooses the count per call when the parts are automatic;
  * n >= 0 uses n tiles and needs 2 or 3 parts per tile.  A tuning knob of
@@ -225,9 +232,17 @@ int ace_set_tail_parts(ace_ctx* ctx, int ntail);
 
 /* The product-DAG launch of the last evaluation call: workgroups per tile,
  * the number of tail tiles that ran one more part, and the kernel (0: the
- * record kernel; 64, 32 or 16: the run interpreter with that many
- * environments per tile).  Instrumentation; no counterpart in the reference. */
+ * record kernel on 64-environment tiles, 1: on 32-environment tiles; 64, 32
+ * or 16: the run interpreter with that many environments per tile).
+ * Instrumentation; no counterpart in the reference. */
 int ace_last_dag_shape(ace_ctx* ctx, int* parts, int64_t* tail_tiles, int* kernel);
+
+/* Product-DAG kernel choice: 0 (default) the record kernel where the basis
+ * allows it (Invariant or class-split EuclideanVector programs, correlation
+ * order <= 4, at most 320 pooled values), else the run interpreter; 1 the
+ * run interpreter always.  A tuning and testing knob of this implementation;
+ * no counterpart in the reference. */
+int ace_set_kernel(ace_ctx* ctx, int mode);
 
 /* Site energies, ProductEvaluator evaluate (evaluator.jl:121-147), batched.
  * E_out[nenv] (Invariant) or E_out[nenv][3] (EuclideanVector).  species

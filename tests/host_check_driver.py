@@ -25,9 +25,9 @@ for args in [(3, 6, 4, "Invariant", 0), (3, 6, 4, "EuclideanVector", 0), (3, 6, 
     msg = C.create_string_buffer(256)
     assert lib.acehost_validate(C.byref(s), msg, 256) == 0, msg.value
     c = synth.coefficients(len(basis), 3)
-    nc = 1 if pk.prop == 0 else 6
+    nc = 1 if pk.prop == 0 else 2 if pk.classed else 6
     leaf = np.zeros(len(pk.leaf_fid) * nc)
-    task = np.zeros(len(pk.tasks) * (2 if nc == 1 else 6))
+    task = np.zeros(len(pk.tasks) * (6 if nc == 6 else 2))
     e0 = np.zeros(6)
     assert lib.acehost_coeffs(C.byref(s), c.ctypes.data_as(f64p), leaf.ctypes.data_as(f64p),
                               task.ctypes.data_as(f64p), e0.ctypes.data_as(f64p)) == 0
@@ -35,6 +35,21 @@ for args in [(3, 6, 4, "Invariant", 0), (3, 6, 4, "EuclideanVector", 0), (3, 6, 
     ref = [PR.eval_terms(t, ct) for t in pk.prog.leaf_terms]
     if nc == 1:
         assert np.max(np.abs(leaf - np.asarray(ref))) < 1e-15
+    elif nc == 2:
+        # class-split: leaf scalar times its class vector = the three complex
+        # coefficients of the componentwise program
+        L = np.asarray(ref).reshape(-1, 3)
+        ok, u, _ = PR.class_vectors(pk.a2b_cols, pk.a2b_vals.reshape(-1, 6).view(np.complex128), pk.row_class)
+        assert ok
+        cls = [PR.term_class(t[0], pk.row_class) for t in pk.prog.leaf_terms]
+        got = (leaf[0::2] + 1j * leaf[1::2])[:, None] * u[cls]
+        assert np.max(np.abs(got - L)) < 1e-15 * max(1.0, np.abs(L).max())
+        # task constants: a task's class is its energy slot's or its G slot's
+        t5 = pk.tasks.reshape(-1, 5)
+        tcls = [-1 - sl if sl < 0 else pk.gslot_class[sl] for sl in t5[:, 2]]
+        Tt = np.asarray([PR.eval_terms(t, ct) if t else np.zeros(3, complex) for t in pk.prog.task_terms])
+        got = (task[0::2] + 1j * task[1::2])[:, None] * u[tcls]
+        assert np.max(np.abs(got - Tt.reshape(-1, 3))) < 1e-15 * max(1.0, np.abs(Tt).max())
     else:
         L = np.asarray(ref).reshape(-1, 3)
         assert np.max(np.abs(leaf - np.stack([L.real, L.imag], -1).reshape(-1))) < 1e-15

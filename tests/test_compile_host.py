@@ -27,6 +27,7 @@ FIELDS = {
     "tasks": ("tasks", np.int32), "task_term_ptr": ("task_term_ptr", np.int32), "term_row": ("term_row", np.int32),
     "term_scale": ("term_scale", np.float64), "term_conj": ("term_conj", np.int32),
     "gslot_ptr": ("gslot_ptr", np.int32), "gslot_list": ("gslot_list", np.int32),
+    "gslot_class": ("gslot_class", np.int32),
     "rec_hdr": ("rec_hdr", np.uint32), "rec_leaf": ("rec_leaf", np.int32), "feat_rows": ("feat_rows", np.int32),
     "feat_ptr": ("feat_ptr", np.int32), "feat_f": ("feat_f", np.int32), "feat_coef": ("feat_coef", np.float64),
     "fgrad_ptr": ("fgrad_ptr", np.int32), "fgrad_feat": ("fgrad_feat", np.int32), "fgrad_f": ("fgrad_f", np.int32),

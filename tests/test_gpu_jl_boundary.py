@@ -86,31 +86,34 @@ def test_order5_basis_vs_oracle(cuda):
 @pytest.mark.parametrize("deg,order,nenv", [(14, 3, 150), (14, 4, 70)])
 def test_invariant_above_160_pooled_values(cuda, deg, order, nenv):
     """One species, degree 14, L 6: 195 pooled values, too many for a
-    64-environment LDS tile; the interpreter runs 32-environment tiles with two
-    task streams per wave.  Order 4 is the model the reference profiles
-    (profile/profile_linearmodel.jl:11-25)."""
+    64-environment LDS tile.  The record kernel runs 32-environment tiles (the
+    two half-waves on one record stream, two leaves each); the run interpreter
+    (forced) runs 32-environment tiles with two task streams per wave.  Order
+    4 is the model the reference profiles (profile/profile_linearmodel.jl:11-25)."""
     print(f"[gpu] npool 195 invariant, order {order}, degree {deg}", flush=True)
     basis, T = basis_for(order, deg, 6)
     c = synth.coefficients(len(basis), 12)
     off, R, _ = synth.workload_envs(2, nenv=nenv, seed=43)
     ev = HIPEvaluator(basis, c, device=0, waves=8)
     b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
-    E, dE = ev.energy_forces(b)
-    E1 = ev.energy(b)
-    torch.cuda.synchronize()
-    assert ev.last_dag_shape()[2] == 32
     E_ref = O.energy(T, c, off, R)
-    assert rel_err(E.cpu().numpy(), E_ref) < E_TOL
-    # the energy-only call sums the energy tasks over another wave assignment
-    assert rel_err(E1.cpu().numpy(), E_ref) < E_TOL
     n = 40 if order == 4 else nenv
     g_ref = O.grad_config(T, c, off[: n + 1], R[: off[n]])
-    assert rel_err(dE.cpu().numpy().T[: off[n]], g_ref) < F_TOL
+    for mode, kern in ((0, 1), (1, 32)):
+        ev.set_kernel(mode)
+        E, dE = ev.energy_forces(b)
+        E1 = ev.energy(b)
+        torch.cuda.synchronize()
+        assert ev.last_dag_shape()[2] == kern
+        assert rel_err(E.cpu().numpy(), E_ref) < E_TOL
+        # the energy-only call sums the energy tasks over another wave assignment
+        assert rel_err(E1.cpu().numpy(), E_ref) < E_TOL
+        assert rel_err(dE.cpu().numpy().T[: off[n]], g_ref) < F_TOL
 
 
 def test_two_species_order4_degree12_vs_oracle(cuda):
     """Two species, order 4, degree 12, L 6 (Invariant): 278 pooled values,
-    491k AA functions, 38.8k basis functions.  The interpreter runs
+    491k AA functions, 38.8k basis functions.  The record kernel runs
     32-environment tiles; 96 environments are three tiles.  The oracle costs
     about 0.7 s per environment for the energy and 4.6 s for the gradient, so
     it checks 16 energies and 3 gradients spread over the tiles."""
@@ -122,7 +125,7 @@ def test_two_species_order4_degree12_vs_oracle(cuda):
     b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
     E, dE = ev.energy_forces(b)
     torch.cuda.synchronize()
-    assert ev.last_dag_shape()[2] == 32
+    assert ev.last_dag_shape()[2] == 1
     E, dE = E.cpu().numpy(), dE.cpu().numpy()
     picks = list(range(0, 96, 6))
     E_ref, g, g_ref = [], [], []

@@ -220,18 +220,41 @@ def test_config3_fcc_forces_fd(cuda):
     assert rel_err(dE.T[j0:j1], fd) < 1e-6
 
 
-def test_vector_property(cuda):
-    """EuclideanVector, order 3, degree 6: 64-environment tiles; 150
-    environments are two full tiles and a partial one."""
-    print("[gpu] EuclideanVector order 3 degree 6, 150 envs", flush=True)
+_VEC_REF = {}
+
+
+@pytest.mark.parametrize("mode,parts,tail", [(0, 0, -1), (0, 2, -1), (0, 3, 1), (1, 0, -1)],
+                         ids=["records", "records-2parts", "records-3parts-tail", "interpreter"])
+def test_vector_property(cuda, mode, parts, tail):
+    """EuclideanVector, order 3, degree 6 (class-split programs): 64-environment
+    tiles; 150 environments are two full tiles and a partial one.  The record
+    kernel with complex leaf coefficients, with the tiles split over 2 and 3
+    workgroups (and one tail tile of 4), and the run interpreter; energies
+    with and without forces."""
+    print(f"[gpu] EuclideanVector order 3 degree 6, 150 envs, kernel mode {mode}, parts {parts}", flush=True)
     basis, T = basis_for(3, 6, 4, prop="EuclideanVector")
     c = synth.coefficients(len(basis), 4)
     off, R, sp = random_batch(150, 11)
-    _, E, dE = _run(basis, T, c, off, R)
-    assert rel_err(E, O.energy(T, c, off, R)) < E_TOL
-    g_ref = O.grad_config(T, c, off, R)              # [nbr, p, d]
+    ev = HIPEvaluator(basis, c, device=0, waves=8)
+    ev.set_kernel(mode)
+    ev.set_parts(parts)
+    ev.set_tail_parts(tail)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    torch.cuda.synchronize()
+    P, ntail, kern = ev.last_dag_shape()
+    assert kern == (0 if mode == 0 else 64)
+    if mode == 0 and parts:
+        assert P == parts and ntail == max(0, tail)
+    E1 = ev.energy(b)
+    E, dE, E1 = E.cpu().numpy(), dE.cpu().numpy(), E1.cpu().numpy()
+    if not _VEC_REF:
+        _VEC_REF["E"] = O.energy(T, c, off, R)
+        _VEC_REF["g"] = O.grad_config(T, c, off, R)   # [nbr, p, d]
+    assert rel_err(E, _VEC_REF["E"]) < E_TOL
+    assert rel_err(E1, _VEC_REF["E"]) < E_TOL
     g = dE.reshape(3, 3, -1).transpose(2, 0, 1)
-    assert rel_err(g, g_ref) < F_TOL
+    assert rel_err(g, _VEC_REF["g"]) < F_TOL
 
 
 def test_two_species_invariant(cuda):
@@ -329,19 +352,31 @@ def test_invalid_inputs_raise(cuda):
         ev.energy(EnvBatch(off2, R.T.copy(), sp, device="cuda:0"))
 
 
-@pytest.mark.parametrize("waves", [8, 16])
-def test_config4_two_species_vector(cuda, waves):
+_C4_REF = {}
+
+
+@pytest.mark.parametrize("waves,mode", [(8, 0), (16, 0), (8, 1)], ids=["records-w8", "records-w16", "interpreter-w8"])
+def test_config4_two_species_vector(cuda, waves, mode):
     """Config 4 (2 species, EuclideanVector, order 3, degree 10, L 5): 220
-    pooled values, 32-environment tiles with two task streams per wave.
-    100 environments are three full tiles and a partial one; every energy
-    and every per-neighbour Jacobian is compared with the oracle."""
-    print(f"[gpu] config4: 100 envs, waves={waves} (32-env tiles)", flush=True)
+    pooled values, 32-environment tiles: the record kernel with the two
+    half-waves on one record stream, or the run interpreter with two task
+    streams per wave.  100 environments are three full tiles and a partial
+    one; every energy and every per-neighbour Jacobian is compared with the
+    oracle."""
+    print(f"[gpu] config4: 100 envs, waves={waves}, kernel mode {mode} (32-env tiles)", flush=True)
     basis, T = basis_for(3, 10, 5, prop="EuclideanVector", species=2)
     c = synth.coefficients(len(basis), 4)
     off, R, sp = synth.workload_envs(4, nenv=100)
-    ev, E, dE = _run(basis, T, c, off, R, sp, waves=waves)
-    assert ev.last_dag_shape()[2] == 32
-    assert rel_err(E, O.energy(T, c, off, R, sp)) < E_TOL
-    g_ref = O.grad_config(T, c, off, R, sp)
+    ev = HIPEvaluator(basis, c, device=0, waves=waves)
+    ev.set_kernel(mode)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    torch.cuda.synchronize()
+    assert ev.last_dag_shape()[2] == (1 if mode == 0 else 32)
+    E, dE = E.cpu().numpy(), dE.cpu().numpy()
+    if not _C4_REF:
+        _C4_REF["E"] = O.energy(T, c, off, R, sp)
+        _C4_REF["g"] = O.grad_config(T, c, off, R, sp)
+    assert rel_err(E, _C4_REF["E"]) < E_TOL
     g = dE.reshape(3, 3, -1).transpose(2, 0, 1)
-    assert rel_err(g, g_ref) < F_TOL
+    assert rel_err(g, _C4_REF["g"]) < F_TOL
