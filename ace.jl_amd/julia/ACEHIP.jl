@@ -42,7 +42,7 @@ export HIPEvaluator, hip_model, evaluate_batch, grad_config_batch
 
 const libacehip = get(ENV, "LIBACEHIP", normpath(joinpath(@__DIR__, "..", "acehip", "libacehip.so")))
 
-const ABI_VERSION = Int32(2)
+const ABI_VERSION = Int32(3)
 const PROP_INVARIANT = Int32(0)
 const PROP_EUCLIDEAN_VECTOR = Int32(1)
 const COMP_CATEGORICAL = Int32(0)

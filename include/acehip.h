@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define ACEHIP_ABI_VERSION 2
+#define ACEHIP_ABI_VERSION 3
 
 typedef struct ace_ctx ace_ctx;
 
