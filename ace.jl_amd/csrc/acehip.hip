@@ -261,14 +261,18 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
             v3 = at(h1.x);
         }
     }
-    if (U1) a1 = at(h1.y & 0x7fffffffu);
-    if (U2) a2 = at(h1.z & 0x7fffffffu);
-    if (U3) a3 = at(h1.w & 0x7fffffffu);
+    if (U1) a1 = at(h1.y);
+    if (U2) a2 = at(h1.z);
+    if (U3) a3 = at(h1.w);
     double2 S = make_double2(0.0, 0.0);
     if (CPLX) {
         if (HALF) {
-            rec_cfma(S, hi ? h4 : h2, v0);
-            rec_cfma(S, hi ? h5 : h3, v1);
+            // per-lane choice of the coefficient words (field by field: a
+            // whole-record select would go through memory)
+            const uint4 c0 = make_uint4(hi ? h4.x : h2.x, hi ? h4.y : h2.y, hi ? h4.z : h2.z, hi ? h4.w : h2.w);
+            const uint4 c1 = make_uint4(hi ? h5.x : h3.x, hi ? h5.y : h3.y, hi ? h5.z : h3.z, hi ? h5.w : h3.w);
+            rec_cfma(S, c0, v0);
+            rec_cfma(S, c1, v1);
         } else {
             rec_cfma(S, h2, v0);
             rec_cfma(S, h3, v1);
@@ -292,19 +296,19 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
             S.y += c2 * v2.y + c3 * v3.y;
         }
     }
-    S.y = flip_sign_if(S.y, ctl >> 16);  // conjugated leaves: conj(sum c A) = sum c conj(A)
+    S.y = flip_sign_if(S.y, ctl >> 16);  // bit 31, conjugated leaves: conj(sum c A) = sum c conj(A)
     if (U1) {
-        a1.y = flip_sign_if(a1.y, h1.y >> 16);
+        a1.y = flip_sign_if(a1.y, ctl >> 13);  // bit 28
         P1 = a1;
         Pc = a1;
     }
     if (U2) {
-        a2.y = flip_sign_if(a2.y, h1.z >> 16);
+        a2.y = flip_sign_if(a2.y, ctl >> 14);  // bit 29
         P2 = cmul(P1, a2);
         Pc = P2;
     }
     if (U3) {
-        a3.y = flip_sign_if(a3.y, h1.w >> 16);
+        a3.y = flip_sign_if(a3.y, ctl >> 15);  // bit 30
         Pc = cmul(P2, a3);
     }
     acc.x += Pc.x * S.x - Pc.y * S.y;
@@ -312,7 +316,7 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
 }
 
 template <bool CPLX, bool HALF>
-__global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
+__global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __restrict__ recs, int64_t nenv,
                                                const double2* __restrict__ Aplus,
                                                int64_t estride, double* __restrict__ E_out,
                                                double2* __restrict__ Gbuf, const double* __restrict__ e0p,
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
         // recomputes its deepest level and Pc is always the last level computed
         // (1 for the prefix-free runs, which come first).
         double2 P1 = make_double2(1.0, 0.0), P2 = P1, Pc = P1;
-        const uint4* rp = reinterpret_cast<const uint4*>(a.recs) + RQ * (int64_t)rb;
+        const uint4* rp = recs + RQ * (int64_t)rb;
         uint4 h0 = rp[0], h1 = rp[1], h2 = rp[2], h3 = rp[3], h4, h5;
         if (CPLX) {
             h4 = rp[4];
@@ -377,7 +381,7 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, int64_t nenv,
 #pragma unroll 2
         for (int r = rb; r < re; ++r) {
             // next record (record nrec exists and is all zeros)
-            const uint4* np = reinterpret_cast<const uint4*>(a.recs) + RQ * (int64_t)(r + 1);
+            const uint4* np = recs + RQ * (int64_t)(r + 1);
             const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
             uint4 n4, n5;
             if (CPLX) {
@@ -1002,12 +1006,12 @@ static void launch_dagr(int ET, dim3 grid, dim3 block, size_t lds, hipStream_t s
                         const double* e0p, const int32_t* env_list, const int32_t* env_count) {
     if (ET == 64) {
         set_func_lds((const void*)k_dagr<CPLX, false>, lds);
-        hipLaunchKernelGGL((k_dagr<CPLX, false>), grid, block, lds, st, ra, nenv, Aplus, estride, E_out, Gbuf, e0p,
-                           env_list, env_count);
+        hipLaunchKernelGGL((k_dagr<CPLX, false>), grid, block, lds, st, ra, reinterpret_cast<const uint4*>(ra.recs),
+                           nenv, Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
     } else {
         set_func_lds((const void*)k_dagr<CPLX, true>, lds);
-        hipLaunchKernelGGL((k_dagr<CPLX, true>), grid, block, lds, st, ra, nenv, Aplus, estride, E_out, Gbuf, e0p,
-                           env_list, env_count);
+        hipLaunchKernelGGL((k_dagr<CPLX, true>), grid, block, lds, st, ra, reinterpret_cast<const uint4*>(ra.recs),
+                           nenv, Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
     }
 }
 

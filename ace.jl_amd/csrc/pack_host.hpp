@@ -400,16 +400,17 @@ inline Coeffs coefficients(const PackCopy& p, const double* c) {
 //   dw0    control: REC_MORE (leaves 2 and 3 present), REC_U1 / REC_U2 /
 //          REC_U3 (prefix level 1 / 2 / 3 is recomputed) -- together the
 //          record type in bits 2-5 --, REC_CONJ (the leaves are conjugated:
-//          one flag per record, since the leaf's m is fixed by the prefix)
+//          one flag per record, since the leaf's m is fixed by the prefix),
+//          REC_CONJ1 / 2 / 3 (prefix factor t is conjugated)
 //   dw1-4  leaf i: LDS byte offset of its pooled value (k << lds_shift: 64
 //          lanes x 16 B at shift 10, 32 lanes at shift 9)
-//   dw5-7  prefix factor t: LDS byte offset | conjugation in bit 31
+//   dw5-7  prefix factor t: LDS byte offset
 //   dw8-15 leaf coefficients (FP64; zero for absent leaves), or
 //   dw8-23 complex leaf coefficients (re, im), stored conjugated when the
 //          leaves are: sum c conj(A) = conj(sum conj(c) A)
 // One zero record follows the last so the kernel can always load record r + 1.
 constexpr uint32_t REC_MORE = 1u << 2, REC_U1 = 1u << 3, REC_U2 = 1u << 4, REC_U3 = 1u << 5;
-constexpr uint32_t REC_CONJ = 1u << 31;
+constexpr uint32_t REC_CONJ1 = 1u << 28, REC_CONJ2 = 1u << 29, REC_CONJ3 = 1u << 30, REC_CONJ = 1u << 31;
 
 inline int rec_words(const PackCopy& p) { return p.classed ? 24 : 16; }
 
@@ -433,9 +434,11 @@ inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint
         for (int i = 0; i < nl; ++i)
             if (((lid[i] & 0x8000u) != 0) != cj) return "record leaves mix conjugation";
         if (cj) ctl |= REC_CONJ;
+        for (int t = 0; t < 3; ++t)
+            if (fid[t] & 0x8000u) ctl |= REC_CONJ1 << t;
         o[0] = ctl;
         for (int i = 0; i < 4; ++i) o[1 + i] = i < nl ? off(lid[i]) : 0u;
-        for (int t = 0; t < 3; ++t) o[5 + t] = off(fid[t]) | ((fid[t] & 0x8000u) << 16);
+        for (int t = 0; t < 3; ++t) o[5 + t] = off(fid[t]);
         if (p.classed) {
             double c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             for (int i = 0; i < 4; ++i) {
