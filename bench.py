@@ -377,7 +377,7 @@ def main():
                             args.epochs, log=lambda m: print("[surrogate]", m, file=sys.stderr, flush=True))
         sur["speedup_vs_plain"] = round(sur["value"] / value, 3)
         out["surrogate"] = sur
-    if not args.no_cpu and ev.prop == 0:
+    if not args.no_cpu:
         T = basis_tables(basis)
         out["cpu_baseline"] = cpu_baseline(T, c, off, R, sp)
     print(json.dumps(out), flush=True)

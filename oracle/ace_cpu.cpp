@@ -1,7 +1,8 @@
 // CPU restatement of ACE.jl's ProductEvaluator — TEST INFRASTRUCTURE ONLY.
 //
-// A C++/OpenMP port of the reference algorithm for the Invariant
-// LinearACEModel, used as the second oracle and as the timed CPU baseline
+// A C++/OpenMP port of the reference algorithm for the Invariant and
+// EuclideanVector LinearACEModel, used as the second oracle and as the timed
+// CPU baseline
 // (cpu_baseline.kind = "port").  It follows ACE.jl v0.12.46 loop by loop:
 //   * Rn: polytransform + OrthPolyBasis evaluate! / evaluate_ed!
 //     (src/transforms/distancetransforms.jl:73; src/polynomials/orthpolys.jl:
@@ -10,9 +11,12 @@
 //     cYlm_ed!, dspher_to_dcart (src/polynomials/sphericalharmonics.jl:43-65,
 //     167-267, 379-443)
 //   * A and dA per neighbour: _add_into_A_dA! (src/product_1pbasis.jl:169-244)
-//   * energy: evaluator.jl:121-147 (naive left-to-right products per AA row)
+//   * energy: evaluator.jl:121-147 (naive left-to-right products per AA row);
+//     EuclideanVector: the three components Re(AA c~_p), c~ complex
+//     (properties.jl:165-248)
 //   * forces: _rrule_evaluate with _AA_local_adjoints! (evaluator.jl:161-200,
-//     pibasis.jl:338-390): dAco, then g_j = sum_iA Re(dAco[iA] dA[iA, j]).
+//     pibasis.jl:338-390): dAco, then g_j = sum_iA Re(dAco[iA] dA[iA, j])
+//     (per component p for EuclideanVector: g[j][p][d]).
 // One environment per OpenMP iteration, schedule(dynamic).
 #include <omp.h>
 
@@ -37,7 +41,8 @@ struct acecpu_tables {
     int32_t nAA, maxorder;
     const int32_t* orders;
     const int32_t* iAA2iA;  // [nAA][maxorder], 0-based
-    const double* ctilde;   // [nAA]
+    const double* ctilde;   // [nAA] (ncomp 1) or [nAA][3][re, im] (ncomp 3)
+    int32_t ncomp;          // 1 Invariant, 3 EuclideanVector
 };
 
 }  // extern "C"
@@ -51,7 +56,7 @@ struct Work {
     std::vector<double> P, dP, R, dR;
     std::vector<cplx> Y;
     std::vector<cplx> dY;  // [nY][3]
-    std::vector<cplx> A, dA, dAco, dAAdA;
+    std::vector<cplx> A, dA, dAco, dAAdA;  // dAco [nA][ncomp]
     std::vector<double> alpA, alpB;
 };
 
@@ -251,19 +256,26 @@ void env_eval(const acecpu_tables& T, int nX, const double* R, const int8_t* sp,
         }
     }
     // energy (evaluator.jl:121-147)
-    double val = 0.0;
+    const int NC = T.ncomp;
+    auto coef = [&](int r, int p) {
+        return NC == 1 ? cplx(T.ctilde[r], 0.0) : cplx(T.ctilde[6 * (size_t)r + 2 * p], T.ctilde[6 * (size_t)r + 2 * p + 1]);
+    };
+    double val[3] = {0.0, 0.0, 0.0};
     int i0 = 0;
     if (T.nAA > 0 && T.orders[0] == 0) {
-        val += T.ctilde[0];
+        for (int p = 0; p < NC; ++p) val[p] += coef(0, p).real();
         i0 = 1;
     }
     for (int r = i0; r < T.nAA; ++r) {
         const int* ix = T.iAA2iA + (size_t)r * T.maxorder;
         cplx aa = w.A[ix[0]];
         for (int t = 1; t < T.orders[r]; ++t) aa *= w.A[ix[t]];
-        val += aa.real() * T.ctilde[r];
+        if (NC == 1)
+            val[0] += aa.real() * T.ctilde[r];
+        else
+            for (int p = 0; p < 3; ++p) val[p] += (aa * coef(r, p)).real();
     }
-    *E = val;
+    for (int p = 0; p < NC; ++p) E[p] = val[p];
     if (!forces) return;
     // dAco (evaluator.jl:174-185, pibasis.jl:338-390)
     std::fill(w.dAco.begin(), w.dAco.end(), cplx(0, 0));
@@ -295,27 +307,34 @@ void env_eval(const acecpu_tables& T, int nX, const double* R, const int8_t* sp,
             bwd *= A2;
             d[0] *= bwd;
         }
-        for (int t = 0; t < ord; ++t) w.dAco[ix[t]] += d[t] * T.ctilde[r];
+        if (NC == 1)
+            for (int t = 0; t < ord; ++t) w.dAco[ix[t]] += d[t] * T.ctilde[r];
+        else
+            for (int t = 0; t < ord; ++t)
+                for (int p = 0; p < 3; ++p) w.dAco[3 * (size_t)ix[t] + p] += d[t] * coef(r, p);
     }
-    for (int j = 0; j < nX; ++j) {
-        double gx = 0, gy = 0, gz = 0;
-        for (int iA = 0; iA < nA; ++iA) {
-            const cplx* da = &w.dA[((size_t)iA * nX + j) * 3];
-            gx += (w.dAco[iA] * da[0]).real();
-            gy += (w.dAco[iA] * da[1]).real();
-            gz += (w.dAco[iA] * da[2]).real();
+    for (int j = 0; j < nX; ++j)
+        for (int p = 0; p < NC; ++p) {
+            double gx = 0, gy = 0, gz = 0;
+            for (int iA = 0; iA < nA; ++iA) {
+                const cplx* da = &w.dA[((size_t)iA * nX + j) * 3];
+                const cplx dc = w.dAco[(size_t)NC * iA + p];
+                gx += (dc * da[0]).real();
+                gy += (dc * da[1]).real();
+                gz += (dc * da[2]).real();
+            }
+            double* gj = g + 3 * ((size_t)NC * j + p);
+            gj[0] = gx;
+            gj[1] = gy;
+            gj[2] = gz;
         }
-        g[3 * j] = gx;
-        g[3 * j + 1] = gy;
-        g[3 * j + 2] = gz;
-    }
 }
 
 }  // namespace
 
 extern "C" int acecpu_eval(const acecpu_tables* T, int64_t nenv, const int32_t* off, const double* R,
                            const int8_t* species, double* E, double* g, int forces, int nthreads) {
-    if (!T || nenv < 0) return -1;
+    if (!T || nenv < 0 || (T->ncomp != 1 && T->ncomp != 3)) return -1;
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel
     {
@@ -328,14 +347,14 @@ extern "C" int acecpu_eval(const acecpu_tables* T, int64_t nenv, const int32_t* 
         w.Y.assign((T->L + 1) * (T->L + 1), cplx(0, 0));
         w.dY.assign(3 * (T->L + 1) * (T->L + 1), cplx(0, 0));
         w.A.assign(T->nA, cplx(0, 0));
-        w.dAco.assign(T->nA, cplx(0, 0));
+        w.dAco.assign((size_t)T->nA * T->ncomp, cplx(0, 0));
         w.dAAdA.assign(T->maxorder + 1, cplx(0, 0));
         alp_coeffs(T->L, w.alpA, w.alpB);
 #pragma omp for schedule(dynamic, 16)
         for (int64_t e = 0; e < nenv; ++e) {
             const int j0 = off[e], nX = off[e + 1] - off[e];
-            env_eval(*T, nX, R + 3 * (size_t)j0, species ? species + j0 : nullptr, forces != 0, w, E + e,
-                     forces ? g + 3 * (size_t)j0 : nullptr);
+            env_eval(*T, nX, R + 3 * (size_t)j0, species ? species + j0 : nullptr, forces != 0, w,
+                     E + (size_t)T->ncomp * e, forces ? g + 3 * (size_t)T->ncomp * j0 : nullptr);
         }
     }
     return 0;

@@ -20,7 +20,7 @@ class Tables(C.Structure):
                 ("L", C.c_int32), ("r0", C.c_double), ("tl", C.c_double), ("tr", C.c_double),
                 ("rA", _f64p), ("rB", _f64p), ("rC", _f64p), ("nA", C.c_int32), ("aspec", _i32p),
                 ("nAA", C.c_int32), ("maxorder", C.c_int32), ("orders", _i32p), ("iAA2iA", _i32p),
-                ("ctilde", _f64p)]
+                ("ctilde", _f64p), ("ncomp", C.c_int32)]
 
 
 def build():
@@ -30,11 +30,10 @@ def build():
 
 
 class CpuModel:
-    """Invariant LinearACEModel on the CPU port; ``T`` = shared basis tables."""
+    """Invariant or EuclideanVector LinearACEModel on the CPU port; ``T`` =
+    shared basis tables.  EuclideanVector: E [nenv, 3], g [nnbr, 3 (p), 3 (d)]."""
 
     def __init__(self, T, c):
-        if T["prop"] != "Invariant":
-            raise ValueError("the CPU port covers the Invariant model")
         if not os.path.exists(LIB):
             build()
         self.lib = C.CDLL(LIB)
@@ -59,7 +58,12 @@ class CpuModel:
         t.maxorder = int(np.asarray(T["iAA2iA"]).shape[1])
         t.orders = arr(T["orders"], np.int32)
         t.iAA2iA = arr(T["iAA2iA"], np.int32)
-        t.ctilde = arr(O.ctilde(T, c), np.float64)
+        ct = O.ctilde(T, c)
+        self.ncomp = 1 if T["prop"] == "Invariant" else 3
+        if self.ncomp == 3:
+            ct = np.stack([ct.real, ct.imag], axis=-1).reshape(-1)
+        t.ctilde = arr(ct, np.float64)
+        t.ncomp = self.ncomp
         self.t = t
 
     def max_threads(self):
@@ -69,8 +73,9 @@ class CpuModel:
         off = np.ascontiguousarray(off, dtype=np.int32)
         R = np.ascontiguousarray(R, dtype=np.float64)
         nenv = len(off) - 1
-        E = np.zeros(nenv)
-        g = np.zeros_like(R) if forces else np.zeros(1)
+        nc = self.ncomp
+        E = np.zeros(nenv) if nc == 1 else np.zeros((nenv, 3))
+        g = (np.zeros_like(R) if nc == 1 else np.zeros((len(R), 3, 3))) if forces else np.zeros(1)
         sp = None if species is None else np.ascontiguousarray(species, dtype=np.int8)
         rc = self.lib.acecpu_eval(C.byref(self.t), nenv, off.ctypes.data_as(_i32p), R.ctypes.data_as(_f64p),
                                   None if sp is None else sp.ctypes.data_as(C.c_void_p),

@@ -51,10 +51,11 @@ def test_cpu_port_matches_oracle():
     from ace_cpu import CpuModel
     from acehip import synth
     from helpers import basis_for, rel_err
-    for args, wid in [((3, 6, 4), 1), ((4, 12, 6), 2)]:
-        basis, T = basis_for(*args)
+    for args, kw, wid in [((3, 6, 4), {}, 1), ((4, 12, 6), {}, 2),
+                          ((3, 10, 5), {"prop": "EuclideanVector", "species": 2}, 4)]:
+        basis, T = basis_for(*args, **kw)
         c = synth.coefficients(len(basis), 1)
-        off, R, _ = synth.workload_envs(wid, nenv=6)
-        E, g = CpuModel(T, c).eval(off, R)
-        assert rel_err(E, O.energy(T, c, off, R)) < 1e-13
-        assert rel_err(g, O.grad_config(T, c, off, R)) < 1e-12
+        off, R, sp = synth.workload_envs(wid, nenv=6)
+        E, g = CpuModel(T, c).eval(off, R, species=sp)
+        assert rel_err(E, O.energy(T, c, off, R, sp)) < 1e-13
+        assert rel_err(g, O.grad_config(T, c, off, R, sp)) < 1e-12
