@@ -1,4 +1,4 @@
-"""Synthetic environments and the five workload definitions.
+"""Synthetic environments and the workload definitions.
 
 Directions are normalized Gaussians (ACE.jl ``rand_sphere``,
 src/utils/auxiliary.jl:6-9); radii uniform in [rmin, rmax] (``rand_radial``,
@@ -10,7 +10,7 @@ import numpy as np
 
 from .coupling import EuclideanVector, Invariant
 from .onep import Categorical1pBasis, Product1pBasis, RnYlm_1pbasis
-from .pibasis import SimpleSparseBasis
+from .pibasis import SimpleSparseBasis, SparseBasis
 from .symmbasis import symmetric_basis
 
 WORKLOADS = {
@@ -24,15 +24,35 @@ WORKLOADS = {
             nenv=65_536, nbr="poisson40", seed=4, species=2),
     5: dict(name="learned-surrogate", order=4, maxdeg=12, L=6, prop="Invariant",
             nenv=1_000_000, nbr="poisson30", seed=5),
+    # size workloads beyond the configurations above
+    # the model of profile/profile_linearmodel.jl:11-25 (SparseBasis, p = 1,
+    # weights n 1, l 1.5; maxL = ceil(14 / 1.5)) on 10 neighbours
+    6: dict(name="reference-profiled-model", order=4, maxdeg=14, L=10, prop="Invariant",
+            nenv=100_000, nbr="fixed10", seed=6, weight={"n": 1.0, "l": 1.5}),
+    # one species, SimpleSparseBasis: 195 pooled values (32-environment tiles)
+    7: dict(name="large-pool-invariant", order=4, maxdeg=14, L=6, prop="Invariant",
+            nenv=100_000, nbr="poisson30", seed=7),
+    # two species: 278 pooled values, 491k AA functions
+    8: dict(name="two-species-order4", order=4, maxdeg=12, L=6, prop="Invariant",
+            nenv=32_768, nbr="poisson30", seed=8, species=2),
+    # correlation order 5 (run interpreter, prefixes of four factors)
+    9: dict(name="order-5", order=5, maxdeg=10, L=4, prop="Invariant",
+            nenv=100_000, nbr="poisson30", seed=9),
 }
 
 
 def make_basis(w: dict):
     """Basis of a workload: RnYlm_1pbasis(maxdeg, maxL, r0=1, rcut=5) (with a
     leading Categorical1pBasis for two species), SimpleSparseBasis(order,
-    maxdeg), Invariant or EuclideanVector(isreal=true)."""
-    Bsel = SimpleSparseBasis(w["order"], w["maxdeg"])
-    RnYlm = RnYlm_1pbasis(maxdeg=w["maxdeg"], maxL=w["L"], r0=1.0, rcut=5.0)
+    maxdeg) -- or SparseBasis(order, p=1, default_maxdeg=maxdeg, weight) when
+    the workload gives weights, the 1p spec then selected by the same
+    weighted degree --, Invariant or EuclideanVector(isreal=true)."""
+    if w.get("weight"):
+        Bsel = SparseBasis(w["order"], p=1, default_maxdeg=w["maxdeg"], weight=w["weight"])
+        RnYlm = RnYlm_1pbasis(maxdeg=w["maxdeg"], maxL=w["L"], Bsel=Bsel, r0=1.0, rcut=5.0)
+    else:
+        Bsel = SimpleSparseBasis(w["order"], w["maxdeg"])
+        RnYlm = RnYlm_1pbasis(maxdeg=w["maxdeg"], maxL=w["L"], r0=1.0, rcut=5.0)
     if w.get("species"):
         cat = Categorical1pBasis(list(range(w["species"])), varsym="mu", idxsym="mu")
         B1p = Product1pBasis((cat,) + RnYlm.bases)
@@ -55,8 +75,8 @@ def rand_dirs(rng, n):
 def random_envs(nenv, nbr, seed, rmin=0.8, rmax=5.0, nspecies=0):
     """Returns (offsets[nenv+1] int32, R[nnbr,3], species[nnbr] int8 | None)."""
     rng = np.random.default_rng(seed)
-    if nbr == "fixed12":
-        nX = np.full(nenv, 12, dtype=np.int64)
+    if nbr.startswith("fixed"):
+        nX = np.full(nenv, int(nbr[len("fixed"):]), dtype=np.int64)
     elif nbr.startswith("poisson"):
         lam = float(nbr[len("poisson"):])
         nX = np.clip(rng.poisson(lam, size=nenv), 5, 80)

@@ -361,7 +361,9 @@ inline RadialParams radial_params(const ace_ctx* ctx) {
         case 5: FN<5>(__VA_ARGS__); break;               \
         case 6: FN<6>(__VA_ARGS__); break;               \
         case 7: FN<7>(__VA_ARGS__); break;               \
-        default: FN<8>(__VA_ARGS__); break;              \
+        case 8: FN<8>(__VA_ARGS__); break;               \
+        case 9: FN<9>(__VA_ARGS__); break;               \
+        default: FN<10>(__VA_ARGS__); break;             \
     }
 
 // ---- launchers shared between units (all stream-ordered; 0 or -1 with the

@@ -1,5 +1,5 @@
 // libacehip.so, forces for Lmax 0-4, and the dispatch over Lmax (the other
-// values are instantiated in ace_force_l5.hip ... ace_force_l8.hip).
+// values are instantiated in ace_force_l5.hip ... ace_force_l10.hip).
 #include "ace_force_impl.hpp"
 
 template int launch_force_l<0>(ace_ctx*, hipStream_t, int64_t, const int32_t*, const double*, int64_t,
@@ -20,6 +20,10 @@ extern template int launch_force_l<7>(ace_ctx*, hipStream_t, int64_t, const int3
                                         int64_t, const int8_t*, int64_t, const Workspace&, double*);
 extern template int launch_force_l<8>(ace_ctx*, hipStream_t, int64_t, const int32_t*, const double*,
                                         int64_t, const int8_t*, int64_t, const Workspace&, double*);
+extern template int launch_force_l<9>(ace_ctx*, hipStream_t, int64_t, const int32_t*, const double*,
+                                        int64_t, const int8_t*, int64_t, const Workspace&, double*);
+extern template int launch_force_l<10>(ace_ctx*, hipStream_t, int64_t, const int32_t*, const double*,
+                                         int64_t, const int8_t*, int64_t, const Workspace&, double*);
 
 int launch_force_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
                      int64_t nnbr, const int8_t* species, int64_t estride, const Workspace& ws, double* dE) {
@@ -36,6 +40,8 @@ int launch_force_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
         case 5: return launch_force_l<5>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
         case 6: return launch_force_l<6>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
         case 7: return launch_force_l<7>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
-        default: return launch_force_l<8>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
+        case 8: return launch_force_l<8>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
+        case 9: return launch_force_l<9>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
+        default: return launch_force_l<10>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, dE);
     }
 }

@@ -331,7 +331,9 @@ int launch_pool_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* o
         case 5: return launch_pool<5>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
         case 6: return launch_pool<6>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
         case 7: return launch_pool<7>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
-        default: return launch_pool<8>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 8: return launch_pool<8>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        case 9: return launch_pool<9>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
+        default: return launch_pool<10>(ctx, st, nenv, off, xyz, nnbr, species, estride, ws, nbr_map);
     }
 }
 
@@ -346,6 +348,8 @@ int launch_poolw_any(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
         case 5: return launch_poolw<5>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
         case 6: return launch_poolw<6>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
         case 7: return launch_poolw<7>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
-        default: return launch_poolw<8>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 8: return launch_poolw<8>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        case 9: return launch_poolw<9>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
+        default: return launch_poolw<10>(ctx, st, nenv, off, xyz, w, nnbr, species, estride);
     }
 }

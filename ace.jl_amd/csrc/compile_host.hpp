@@ -172,7 +172,7 @@ inline std::string validate_jl(const ace_jl_basis* b) {
     if (b->abi_version != ACEHIP_ABI_VERSION) return "ABI version mismatch";
     if (b->prop != ACE_PROP_INVARIANT && b->prop != ACE_PROP_EUCLIDEAN_VECTOR) return "unknown property";
     if (b->nr < 1 || b->nr > 32 || !b->A || !b->B || !b->C) return "radial basis: need 1 to 32 functions";
-    if (b->maxL < 0 || b->maxL > 8) return "maxL must be in [0, 8]";
+    if (b->maxL < 0 || b->maxL > 10) return "maxL must be in [0, 10]";
     if (b->ncategories < 0 || b->ncategories > 127) return "ncategories out of range";
     if (b->tp < 1 || b->tp > 16 || b->pl < 0 || b->pr < 0 || b->pl > 16 || b->pr > 16)
         return "transform or envelope powers out of range";

@@ -121,7 +121,7 @@ inline std::string validate_pack(const ace_pack* pk) {
     if (!pk) return "null pack";
     if (pk->abi_version != ACEHIP_ABI_VERSION) return "ABI version mismatch";
     if (pk->prop != ACE_PROP_INVARIANT && pk->prop != ACE_PROP_EUCLIDEAN_VECTOR) return "unknown property";
-    if (pk->L < 0 || pk->L > 8) return "L must be in [0, 8]";
+    if (pk->L < 0 || pk->L > 10) return "L must be in [0, 10]";
     if (pk->nr < 1 || pk->nr > 32) return "nr must be in [1, 32]";
     if (pk->npool < 1 || pk->npool > 512) return "npool out of range (at most 512 pooled functions)";
     if (pk->nspecies < 0 || pk->nspecies > 127) return "nspecies out of range";

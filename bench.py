@@ -38,6 +38,14 @@ WORKLOAD_NAMES = {
        "Lmax 4, Invariant, FP64",
     4: "plain ACE energy+forces, 65,536 envs, 2 species, nX~Poisson(40) clipped [5,80], order 3, degree 10, "
        "Lmax 5, EuclideanVector, FP64",
+    6: "plain ACE energy+forces, 100k envs x 10 neighbours, the model of profile/profile_linearmodel.jl "
+       "(SparseBasis order 4, degree 14, weights n 1 / l 1.5, maxL 10), Invariant, FP64",
+    7: "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], order 4, degree 14, Lmax 6, "
+       "Invariant (195 pooled values), FP64",
+    8: "plain ACE energy+forces, 32,768 envs, 2 species, nX~Poisson(30) clipped [5,80], order 4, degree 12, "
+       "Lmax 6, Invariant (278 pooled values), FP64",
+    9: "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], order 5, degree 10, Lmax 4, "
+       "Invariant, FP64",
 }
 TRAFFIC_SOURCE = "profiles/plain_ef_config2_kernels.md"
 
@@ -231,9 +239,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4),
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 6, 7, 8, 9),
                     help="workload: 2 (default) is the one the metric is quoted on; 1, 3 and 4 are the "
-                         "other configurations, timed with the same step for their profiles")
+                         "other configurations, 6-9 larger bases (the reference's profiled model, 195 and "
+                         "278 pooled values, correlation order 5), timed with the same step")
     ap.add_argument("--nenv", type=int, default=0, help="environments (0: the workload's own size)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--waves", type=int, default=0, help="tile waves (0 = autotune)")

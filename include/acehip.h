@@ -159,7 +159,7 @@ int ace_ctx_create(const ace_pack* pack, int device, const double* c, int64_t nB
  *  - radial basis: OrthPolyBasis fields pl, tl, pr, tr, A, B, C
  *    (orthpolys.jl:82-99) and the transform t = ((1 + r0)/(1 + r))^tp
  *    (PolyTransform, distancetransforms.jl:16,73);
- *  - maxL: the degree bound of the SHBasis (Ylm component);
+ * *  - maxL: the degree bound of the SHBasis (Ylm component), at most 10;
  *  - ncategories: length of the Categorical1pBasis (0 when absent);
  *  - Product1pBasis.indices (product_1pbasis.jl:5-8), Vector{NTuple{ncomp,
  *    Int}} -> indices[nA][ncomp], with comp_kind[c] the kind of bases[c]:

@@ -59,7 +59,7 @@ for args in [(3, 6, 4, "Invariant", 0), (3, 6, 4, "EuclideanVector", 0), (3, 6, 
     assert lib.acehost_lookup(C.byref(s), kidx.ctypes.data_as(C.c_void_p), a4.ctypes.data_as(C.c_void_p)) == 0
     assert np.all(a4.reshape(-1, 4) == np.asarray(T["aspec"]))
     # corrupted packs are rejected with a message, never read out of bounds
-    for field, bad, expect in [("npool", 0, b"npool"), ("L", 9, b"L must"), ("maxorder", 9, b"maxorder"),
+    for field, bad, expect in [("npool", 0, b"npool"), ("L", 11, b"L must"), ("maxorder", 9, b"maxorder"),
                                ("ngslots", 0, b"task out of range")]:
         s2, keep2 = _lib.make_pack(pk)
         setattr(s2, field, bad)

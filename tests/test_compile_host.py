@@ -101,7 +101,7 @@ def test_bad_tables_are_rejected(shim):
     _, T = basis_for(3, 6, 4)
     msg = C.create_string_buffer(256)
     for field, bad, expect in [("maxorder", 9, b"correlation order"), ("nr", 0, b"radial"),
-                               ("ncomp", 1, b"Product1pBasis"), ("maxL", 9, b"maxL")]:
+                               ("ncomp", 1, b"Product1pBasis"), ("maxL", 11, b"maxL")]:
         s, keep = _lib.make_jl_basis(T)
         setattr(s, field, bad)
         assert not shim.acehost_compile(C.byref(s), msg, 256)

@@ -88,8 +88,7 @@ def test_invariant_above_160_pooled_values(cuda, deg, order, nenv):
     """One species, degree 14, L 6: 195 pooled values, too many for a
     64-environment LDS tile.  The record kernel runs 32-environment tiles (the
     two half-waves on one record stream, two leaves each); the run interpreter
-    (forced) runs 32-environment tiles with two task streams per wave.  Order
-    4 is the model the reference profiles (profile/profile_linearmodel.jl:11-25)."""
+    (forced) runs 32-environment tiles with two task streams per wave."""
     print(f"[gpu] npool 195 invariant, order {order}, degree {deg}", flush=True)
     basis, T = basis_for(order, deg, 6)
     c = synth.coefficients(len(basis), 12)
@@ -191,3 +190,45 @@ def test_host_memory_entry_points(cuda, args, kw):
     from acehip._lib import AceError
     with pytest.raises(AceError, match="non-decreasing"):
         check(lib.ace_energy_host(ev.ctx, 70, p(bad), p(xyz), p(spa), p(E1)))
+
+
+@pytest.mark.parametrize("wid,nenv", [(6, 70)], ids=["reference-profiled-model"])
+def test_lmax10_reference_profiled_model(cuda, wid, nenv):
+    """The model of profile/profile_linearmodel.jl:11-25: SparseBasis order
+    4, degree 14 with weights n 1 / l 1.5, maxL = 10 (per-neighbour kernels at
+    Lmax 10), on 10 neighbours; 70 environments (two tiles) against the
+    oracle, gradients on the first 12."""
+    print("[gpu] reference profiled model, Lmax 10", flush=True)
+    w = synth.WORKLOADS[wid]
+    basis = synth.make_basis(w)
+    from acehip.tables import basis_tables
+    T = basis_tables(basis)
+    assert T["L"] == 10
+    c = synth.coefficients(len(basis), 31)
+    off, R, sp = synth.workload_envs(wid, nenv=nenv, seed=32)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    for compiler in ("native", "python"):
+        ev = HIPEvaluator(basis, c, device=0, waves=8, compiler=compiler)
+        E, dE = ev.energy_forces(b)
+        E1 = ev.energy(b)
+        torch.cuda.synchronize()
+        E, dE, E1 = E.cpu().numpy(), dE.cpu().numpy(), E1.cpu().numpy()
+        E_ref = O.energy(T, c, off, R)
+        assert rel_err(E, E_ref) < E_TOL
+        assert rel_err(E1, E_ref) < E_TOL
+        n = 12
+        assert rel_err(dE.T[: off[n]], O.grad_config(T, c, off[: n + 1], R[: off[n]])) < F_TOL
+
+
+def test_lmax9_basis(cuda):
+    """Order 2, degree 9, Lmax 9: the Lmax-9 kernels against the oracle."""
+    print("[gpu] order 2, degree 9, Lmax 9", flush=True)
+    basis, T = basis_for(2, 9, 9)
+    c = synth.coefficients(len(basis), 33)
+    off, R, _ = random_batch(40, 34)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    torch.cuda.synchronize()
+    assert rel_err(E.cpu().numpy(), O.energy(T, c, off, R)) < E_TOL
+    assert rel_err(dE.cpu().numpy().T, O.grad_config(T, c, off, R)) < F_TOL
