@@ -63,7 +63,7 @@ class AcePack(C.Structure):
 
 
 # every symbol declared in include/acehip.h
-EXPORTS = ("ace_ctx_create", "ace_ctx_create_jl", "ace_compile_check", "ace_last_dag_shape", "ace_set_kernel", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_set_tail_parts", "ace_energy",
+EXPORTS = ("ace_ctx_create", "ace_ctx_create_jl", "ace_compile_check", "ace_last_dag_shape", "ace_set_kernel", "ace_set_pool_shape", "ace_set_params", "ace_set_tile", "ace_set_parts", "ace_set_tail_parts", "ace_energy",
            "ace_energy_forces", "ace_basis_A", "ace_basis_AA", "ace_basis_B",
            "ace_basis_dB", "ace_surr_features", "ace_surr_energy_forces",
            "ace_last_timing", "ace_set_validation", "ace_last_error",
@@ -111,6 +111,7 @@ def load(path: str | None = None):
     lib.ace_last_timing.argtypes = [vp, _f64p, _f64p, _f64p]
     lib.ace_set_validation.argtypes = [vp, C.c_int]
     lib.ace_set_kernel.argtypes = [vp, C.c_int]
+    lib.ace_set_pool_shape.argtypes = [vp, C.c_int, C.c_int]
     lib.ace_last_error.restype = C.c_char_p
     lib.ace_last_error.argtypes = []
     lib.ace_ctx_destroy.argtypes = [vp]

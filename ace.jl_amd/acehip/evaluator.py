@@ -81,6 +81,8 @@ class HIPEvaluator:
         self.nB = int(self.T["a2b"]["shape"][0])
         self.nA = len(self.T["aspec"])
         self.nAA = len(self.T["orders"])
+        # pooled (m >= 0) 1p functions: distinct (mu, n, l, |m|)
+        self.npool = len({(int(a[0]), int(a[1]), int(a[2]), abs(int(a[3]))) for a in self.T["aspec"]})
         self.device = device
         c = np.ascontiguousarray(c, dtype=np.float64)
         ctx = C.c_void_p()
@@ -125,6 +127,12 @@ class HIPEvaluator:
     def set_parts(self, parts: int = 0):
         """Workgroups per product-DAG tile: 0 chooses per call (default), else 1 to 4."""
         _lib.check(self.lib.ace_set_parts(self.ctx, int(parts)))
+
+    def set_pool_shape(self, chunk: int = 0, envs_per_wave: int = 0):
+        """Pooling launch: neighbours per chunk (8, 16, 32; 0 = 16) and
+        environments per wave (0 automatic, 1 or 2; two only for bases of at
+        most 32 pooled values)."""
+        _lib.check(self.lib.ace_set_pool_shape(self.ctx, int(chunk), int(envs_per_wave)))
 
     def set_kernel(self, mode: int = 0):
         """Product-DAG kernel: 0 the record kernel where the basis allows it

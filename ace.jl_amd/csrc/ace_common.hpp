@@ -230,6 +230,9 @@ struct ace_ctx {
     // force_interp: run the interpreter even so (ace_set_kernel)
     int rec_et = 0;
     bool force_interp = false;
+    // pooling launch shape (ace_set_pool_shape): neighbours per chunk (8, 16,
+    // 32; 0 = 16) and environments per wave (0 = automatic, 1 = one)
+    int pool_ch = 0, pool_epw = 0;
     DevBuf<double2> cls_u;        // [3 classes][3 components]
     DevBuf<int32_t> gslot_cls;    // [ngslots] class of every G slot
     acehost::PackCopy host;

@@ -7,12 +7,13 @@
 #include "ace_common.hpp"
 
 // k_pool: A+[k][env].  EPW environments per wave, one per group of
-// LPE = 64 / EPW lanes (EPW = 2 when npool <= 32, so that small bases keep
-// the lanes busy).  A group stages its environment's neighbours POOL_CH at a
-// time in its own LDS region ([slot][R_n | Y_lm | species]); its lanes then
-// turn into pooled functions k = hl + LPE p (p < KP) and sum over the staged
-// neighbours in order, holding the sums in registers across chunks.
-template <int L, int EPW, int KP>
+// LPE = 64 / EPW lanes (EPW = 2 possible when npool <= 32, so that small
+// bases keep the lanes busy).  A group stages its environment's neighbours CH
+// at a time (CH <= LPE) in its own LDS region ([slot][R_n | Y_lm | species]);
+// its lanes then turn into pooled functions k = hl + LPE p (p < KP) and sum
+// over the staged neighbours in order, holding the sums in registers across
+// chunks.
+template <int L, int EPW, int KP, int CH>
 __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
                                               const int32_t* __restrict__ off,
                                               const double* __restrict__ xyz, int64_t nnbr,
@@ -23,7 +24,8 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
     constexpr int NP = (L + 1) * (L + 2) / 2;
     constexpr int LPE = 64 / EPW;                 // lanes per environment
     static_assert(EPW == 1 || KP == 1, "two environments per wave take one pooled function per lane");
-    extern __shared__ double smem[];  // [EPW][POOL_CH][rstride]
+    static_assert(CH <= LPE, "a chunk holds at most one neighbour per lane of the group");
+    extern __shared__ double smem[];  // [EPW][CH][rstride]
     const int lane = threadIdx.x;
     const int hl = lane % LPE;
     const int nr = rp.nr;
@@ -32,7 +34,7 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
     const int64_t env = xcd_env(blockIdx.x, nblk) * EPW + lane / LPE;
     const bool valid = env < nenv;
     const int j0 = valid ? off[env] : 0, j1 = valid ? off[env + 1] : 0;
-    double* const sgrp = smem + (lane / LPE) * POOL_CH * rstride;
+    double* const sgrp = smem + (lane / LPE) * CH * rstride;
     // this lane's pooled functions k = hl + LPE p: slot offsets of R_n and Y_lm
     int kr[KP], ky[KP], kmu[KP];
     double ar[KP], ai[KP];
@@ -48,8 +50,8 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
         }
         ar[p] = ai[p] = 0.0;
     }
-    for (int jb = j0; jb < j1; jb += POOL_CH) {
-        const int nch = min(POOL_CH, j1 - jb);
+    for (int jb = j0; jb < j1; jb += CH) {
+        const int nch = min(CH, j1 - jb);
         if (hl < nch) {
             const int j = jb + hl;
             if (nbr_env) nbr_env[j] = (int32_t)env;  // neighbour -> environment, for the flat force kernel
@@ -265,14 +267,38 @@ __global__ __launch_bounds__(64) void k_poolw(RadialParams rp, int64_t nenv, con
         if (kmu[p] != -2) dAw[(int64_t)(lane + 64 * p) * estride + env] = make_double2(ar[p], ai[p]);
 }
 
+template <int L, int EPW, int KP, int CH>
+static void launch_pool_shape(const dim3& grid, size_t lds, hipStream_t st, ace_ctx* ctx, int64_t nenv,
+                              const int32_t* off, const double* xyz, int64_t nnbr, const int8_t* sp,
+                              const Workspace& ws, int64_t estride, int32_t* nbr_env) {
+    set_func_lds((const void*)k_pool<L, EPW, KP, CH>, lds);
+    hipLaunchKernelGGL((k_pool<L, EPW, KP, CH>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr,
+                       sp, ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
+}
+
+template <int L, int EPW, int KP>
+static void launch_pool_ch(int ch, const dim3& grid, size_t lds, hipStream_t st, ace_ctx* ctx, int64_t nenv,
+                           const int32_t* off, const double* xyz, int64_t nnbr, const int8_t* sp, const Workspace& ws,
+                           int64_t estride, int32_t* nbr_env) {
+    if (ch == 8)
+        launch_pool_shape<L, EPW, KP, 8>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (ch == 32)
+        launch_pool_shape<L, EPW, KP, 32>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else
+        launch_pool_shape<L, EPW, KP, 16>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+}
+
 template <int L>
 static int launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off,
                        const double* xyz, int64_t nnbr, const int8_t* species, int64_t estride, const Workspace& ws,
                        bool nbr_map) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
-    // two environments per wave when a half-wave covers the pooled functions
-    const int epw = ctx->npool <= 32 ? 2 : 1;
-    const size_t lds = (size_t)epw * POOL_CH * (ctx->nr + 2 * NP + 1) * sizeof(double);
+    // environments per wave: two when a half-wave covers the pooled functions
+    // (unless the context asks for one), else one; neighbours per chunk: the
+    // context's choice, 16 by default
+    const int epw = ctx->npool <= 32 && ctx->pool_epw != 1 ? 2 : 1;
+    const int ch = ctx->pool_ch > 0 ? ctx->pool_ch : POOL_CH;
+    const size_t lds = (size_t)epw * ch * (ctx->nr + 2 * NP + 1) * sizeof(double);
     int32_t* nbr_env = nullptr;
     if (nbr_map && nnbr > 0) {
         if (!ws.nbr_env || ws.nbr_cap < (size_t)nnbr) return set_err("neighbour map buffer too small");
@@ -282,20 +308,14 @@ static int launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t
     const dim3 grid((unsigned)((nenv + epw - 1) / epw));
     const int8_t* sp = ctx->nspecies > 0 ? species : nullptr;
     // pooled functions per lane: 1 (two environments per wave), 4 or 8
-    if (epw == 2) {
-        set_func_lds((const void*)k_pool<L, 2, 1>, lds);
-        hipLaunchKernelGGL((k_pool<L, 2, 1>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
-                           ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
-    } else if (ctx->npool <= 256) {
-        set_func_lds((const void*)k_pool<L, 1, 4>, lds);
-        hipLaunchKernelGGL((k_pool<L, 1, 4>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
-                           ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
-    } else {
-        if (ctx->npool > 512) return set_err("pooled 1p basis too large (npool > 512)");
-        set_func_lds((const void*)k_pool<L, 1, 8>, lds);
-        hipLaunchKernelGGL((k_pool<L, 1, 8>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr, sp,
-                           ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
-    }
+    if (epw == 2)
+        launch_pool_ch<L, 2, 1>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (ctx->npool <= 256)
+        launch_pool_ch<L, 1, 4>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (ctx->npool <= 512)
+        launch_pool_ch<L, 1, 8>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else
+        return set_err("pooled 1p basis too large (npool > 512)");
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -876,6 +876,17 @@ extern "C" int ace_set_tile(ace_ctx* ctx, int W) {
     return set_err("ace_set_tile: waves per tile not in the pack");
 }
 
+extern "C" int ace_set_pool_shape(ace_ctx* ctx, int chunk, int envs_per_wave) {
+    if (!ctx) return set_err("ace_set_pool_shape: null context");
+    if (chunk != 0 && chunk != 8 && chunk != 16 && chunk != 32)
+        return set_err("ace_set_pool_shape: chunk must be 0 (default 16), 8, 16 or 32");
+    if (envs_per_wave != 0 && envs_per_wave != 1 && envs_per_wave != 2)
+        return set_err("ace_set_pool_shape: envs_per_wave must be 0 (automatic), 1 or 2");
+    ctx->pool_ch = chunk;
+    ctx->pool_epw = envs_per_wave;
+    return 0;
+}
+
 extern "C" int ace_set_kernel(ace_ctx* ctx, int mode) {
     if (!ctx) return set_err("ace_set_kernel: null context");
     if (mode != 0 && mode != 1) return set_err("ace_set_kernel: mode must be 0 (automatic) or 1 (run interpreter)");

@@ -360,6 +360,11 @@ def main():
                    # 0: chosen per call (parts from the tile and CU counts, and
                    # one more part for the tiles of the grid's last round)
                    "tile_parts": ("automatic" if tuned is None or tuned[0][1] == 0 else tuned[0][1]),
+                   "pool_chunk": None if tuned is None else tuned[2][0][0],
+                   "pool_envs_per_wave": None if tuned is None else ("automatic" if tuned[2][0][1] == 0
+                                                                      else tuned[2][0][1]),
+                   "autotune_pool_ms": None if tuned is None else {f"ch{k[0]}{'' if k[1] == 0 else 'x1env'}":
+                                                                   round(v, 4) for k, v in tuned[2][1].items()},
                    "autotune_ms": None if tuned is None else {f"{k[0]}x{k[1] or 'auto'}": round(v, 3)
                                                               for k, v in tuned[1].items()}},
         "roofline": {"bound": "fp64-vector", "compute": "FP64 vector ALU (no MFMA); MI355X FP64 vector peak",

@@ -244,6 +244,12 @@ int ace_last_dag_shape(ace_ctx* ctx, int* parts, int64_t* tail_tiles, int* kerne
  * no counterpart in the reference. */
 int ace_set_kernel(ace_ctx* ctx, int mode);
 
+/* Pooling launch shape: neighbours staged per chunk (8, 16 or 32; 0 = 16)
+ * and environments per wave (0 = automatic: two for bases of at most 32
+ * pooled values, else one; 1 = one).  Results do not depend on it.  A tuning
+ * knob of this implementation; no counterpart in the reference. */
+int ace_set_pool_shape(ace_ctx* ctx, int chunk, int envs_per_wave);
+
 /* Site energies, ProductEvaluator evaluate (evaluator.jl:121-147), batched.
  * E_out[nenv] (Invariant) or E_out[nenv][3] (EuclideanVector).  species
  * [nnbr] int8 category index (NULL without a categorical 1p basis). */

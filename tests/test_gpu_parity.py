@@ -380,3 +380,27 @@ def test_config4_two_species_vector(cuda, waves, mode):
     assert rel_err(E, _C4_REF["E"]) < E_TOL
     g = dE.reshape(3, 3, -1).transpose(2, 0, 1)
     assert rel_err(g, _C4_REF["g"]) < F_TOL
+
+
+@pytest.mark.parametrize("args,wid,nenv", [((3, 6, 4), 1, 300), ((4, 12, 6), 2, 200)], ids=["config1", "config2-small"])
+def test_pool_shapes_bit_identical(cuda, args, wid, nenv):
+    """Every pooling launch shape (8, 16 or 32 neighbours per chunk; one or
+    two environments per wave for bases of at most 32 pooled values) sums the
+    neighbours in the same order: energies and forces are bit-identical to the
+    default shape, and match the oracle."""
+    print(f"[gpu] pooling shapes {args}", flush=True)
+    basis, T = basis_for(*args)
+    c = synth.coefficients(len(basis), 9)
+    off, R, _ = synth.workload_envs(wid, nenv=nenv, seed=17)
+    ev = HIPEvaluator(basis, c, device=0, waves=8)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E0, dE0 = ev.energy_forces(b)
+    E0, dE0 = E0.clone(), dE0.clone()
+    shapes = [(ch, 0) for ch in (8, 16, 32)] + ([(ch, 1) for ch in (8, 16, 32)] if ev.npool <= 32 else [])
+    for ch, epw in shapes:
+        ev.set_pool_shape(ch, epw)
+        E, dE = ev.energy_forces(b)
+        torch.cuda.synchronize()
+        assert torch.equal(E, E0) and torch.equal(dE, dE0), (ch, epw)
+    n = 16
+    assert rel_err(E0.cpu().numpy()[:n], O.energy(T, c, off[: n + 1], R[: off[n]])) < E_TOL
