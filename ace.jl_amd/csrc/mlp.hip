@@ -363,15 +363,22 @@ __global__ void k_to_bf16(int nmemb, const float* __restrict__ params, bf16* __r
     wb[i] = (bf16)v;
 }
 
-// MSE loss gradient on normalized targets: dy = 2 (y - t) / n, loss partial sums
+// MSE loss gradient on normalized targets: dy = 2 (y - t) / n; the loss of
+// member blockIdx.y as wave sums, one atomic add per wave (the loss is a
+// diagnostic: its summation order is not fixed)
 __global__ void k_mse(int64_t nrows, int nmemb, const float* __restrict__ y, const float* __restrict__ t,
                       float* __restrict__ dy, float* __restrict__ loss) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrows * nmemb) return;
-    const int64_t r = i % nrows;
-    const float e = y[i] - t[r];
-    dy[i] = 2.0f * e / (float)nrows;
-    atomicAdd(&loss[i / nrows], e * e / (float)nrows);
+    const int m = blockIdx.y;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float s = 0.0f;
+    if (r < nrows) {
+        const int64_t i = (int64_t)m * nrows + r;
+        const float e = y[i] - t[r];
+        dy[i] = 2.0f * e / (float)nrows;
+        s = e * e / (float)nrows;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&loss[m], s);
 }
 
 // ---------------------------------------------------------------------------
@@ -499,9 +506,8 @@ static int fwd_bwd(acemlp_ctx* c, int64_t nrows, const bf16* X, const float* dy_
     const float* dy = dy_ext;
     if (target) {
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (int64_t)c->nmemb, 0.0f, c->loss.p);
-        const int64_t n = nrows * c->nmemb;
-        hipLaunchKernelGGL(k_mse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nrows, c->nmemb, c->y.p,
-                           target, c->dy.p, c->loss.p);
+        hipLaunchKernelGGL(k_mse, dim3((unsigned)((nrows + 255) / 256), (unsigned)c->nmemb), dim3(256), 0, st, nrows,
+                           c->nmemb, c->y.p, target, c->dy.p, c->loss.p);
         HIP_TRY(hipGetLastError());
         dy = c->dy.p;
     }
