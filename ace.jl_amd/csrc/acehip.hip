@@ -209,8 +209,9 @@ __global__ __launch_bounds__(1024) void k_dag(DagArgs a, int64_t nenv,
 //   HALF = false: 64 environments per tile, lane = environment.
 //   HALF = true:  32 environments per tile (pooled bases of 161-320 values):
 //                 lanes e and e + 32 hold environment e and run the same
-//                 record stream, the low half on leaves 0-1, the high half on
-//                 leaves 2-3; the halves' sums meet at the end of each task.
+//                 record stream, the low half on leaves 0 and 2, the high half
+//                 on leaves 1 and 3; the halves' sums meet at the end of each
+//                 task.
 
 struct DagRecArgs {
     const uint32_t* recs;      // [nrec + 1][16 | 24]
@@ -228,6 +229,11 @@ struct DagRecArgs {
     const int32_t* wtask_ptr_hi;  // [(parts + 1) * W + 1]
     const int32_t* wtask_list_hi;
 };
+
+// Per-lane choice between two record words (the half-wave mode), taken by
+// value: a conditional between the fields themselves is an lvalue, a choice
+// of address, which can put the record registers in scratch memory.
+__device__ __forceinline__ uint32_t pick(bool hi, uint32_t lo_w, uint32_t hi_w) { return hi ? hi_w : lo_w; }
 
 __device__ __forceinline__ double rec_coef(uint32_t lo, uint32_t hi) {
     return __hiloint2double((int)hi, (int)lo);
@@ -250,9 +256,11 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
     auto at = [&](uint32_t off) { return *reinterpret_cast<const double2*>(lds + off); };
     double2 v0, v1, v2, v3, a1, a2, a3;
     if (HALF) {
-        // absent leaves have offset 0 and coefficient 0
-        v0 = at(hi ? h0.w : h0.y);
-        v1 = at(hi ? h1.x : h0.z);
+        // low half: leaves 0 and 2, high half: leaves 1 and 3 (records of at
+        // most two leaves take one per half); absent leaves have offset 0 and
+        // coefficient 0
+        v0 = at(pick(hi, h0.y, h0.z));
+        if (MORE) v1 = at(pick(hi, h0.w, h1.x));
     } else {
         v0 = at(h0.y);
         v1 = at(h0.z);
@@ -269,10 +277,14 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
         if (HALF) {
             // per-lane choice of the coefficient words (field by field: a
             // whole-record select would go through memory)
-            const uint4 c0 = make_uint4(hi ? h4.x : h2.x, hi ? h4.y : h2.y, hi ? h4.z : h2.z, hi ? h4.w : h2.w);
-            const uint4 c1 = make_uint4(hi ? h5.x : h3.x, hi ? h5.y : h3.y, hi ? h5.z : h3.z, hi ? h5.w : h3.w);
+            const uint4 c0 = make_uint4(pick(hi, h2.x, h3.x), pick(hi, h2.y, h3.y), pick(hi, h2.z, h3.z),
+                                        pick(hi, h2.w, h3.w));
             rec_cfma(S, c0, v0);
-            rec_cfma(S, c1, v1);
+            if (MORE) {
+                const uint4 c1 = make_uint4(pick(hi, h4.x, h5.x), pick(hi, h4.y, h5.y), pick(hi, h4.z, h5.z),
+                                            pick(hi, h4.w, h5.w));
+                rec_cfma(S, c1, v1);
+            }
         } else {
             rec_cfma(S, h2, v0);
             rec_cfma(S, h3, v1);
@@ -282,10 +294,14 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
             }
         }
     } else if (HALF) {
-        const double c0 = hi ? rec_coef(h3.x, h3.y) : rec_coef(h2.x, h2.y);
-        const double c1 = hi ? rec_coef(h3.z, h3.w) : rec_coef(h2.z, h2.w);
-        S.x = c0 * v0.x + c1 * v1.x;
-        S.y = c0 * v0.y + c1 * v1.y;
+        const double c0 = rec_coef(pick(hi, h2.x, h2.z), pick(hi, h2.y, h2.w));
+        S.x = c0 * v0.x;
+        S.y = c0 * v0.y;
+        if (MORE) {
+            const double c1 = rec_coef(pick(hi, h3.x, h3.z), pick(hi, h3.y, h3.w));
+            S.x += c1 * v1.x;
+            S.y += c1 * v1.y;
+        }
     } else {
         const double c0 = rec_coef(h2.x, h2.y), c1 = rec_coef(h2.z, h2.w);
         S.x = c0 * v0.x + c1 * v1.x;
