@@ -17,30 +17,36 @@ namespace {
 // ---------------------------------------------------------------------------
 // k_force: +dE/dr_j = Re sum_k G[k] grad(R_n Y_lm)(r_j)
 
-// Component p of G[k] for one environment: the sum over k's slots.  With
+// The NC components of G[k] for one environment: sums over k's slots.  With
 // class-split programs (gcls set) a slot holds one complex value of class
-// gcls[slot], and component p is sum_slots cls_u[class][p] * value.
+// gcls[slot] (read once), and component p is sum_slots cls_u[class][p] * value.
 template <int NC>
-__device__ __forceinline__ double2 gather_g(const int32_t* __restrict__ gslot_ptr,
-                                            const int32_t* __restrict__ gslot_list,
-                                            const int32_t* __restrict__ gcls, const double2* __restrict__ cls_u,
-                                            const double2* __restrict__ Gbuf, int64_t estride, int64_t env, int k,
-                                            int p) {
-    double2 g = make_double2(0.0, 0.0);
+__device__ __forceinline__ void gather_g(const int32_t* __restrict__ gslot_ptr,
+                                         const int32_t* __restrict__ gslot_list,
+                                         const int32_t* __restrict__ gcls, const double2* __restrict__ cls_u,
+                                         const double2* __restrict__ Gbuf, int64_t estride, int64_t env, int k,
+                                         double2 (&g)[NC]) {
+#pragma unroll
+    for (int p = 0; p < NC; ++p) g[p] = make_double2(0.0, 0.0);
     for (int s = gslot_ptr[k]; s < gslot_ptr[k + 1]; ++s) {
         const int slot = gslot_list[s];
         if (NC > 1 && gcls) {
             const double2 v = Gbuf[(int64_t)slot * estride + env];
-            const double2 u = cls_u[3 * gcls[slot] + p];
-            g.x += u.x * v.x - u.y * v.y;
-            g.y += u.x * v.y + u.y * v.x;
+            const double2* u = cls_u + 3 * gcls[slot];
+#pragma unroll
+            for (int p = 0; p < NC; ++p) {
+                g[p].x += u[p].x * v.x - u[p].y * v.y;
+                g[p].y += u[p].x * v.y + u[p].y * v.x;
+            }
         } else {
-            const double2 v = Gbuf[((int64_t)slot * NC + p) * estride + env];
-            g.x += v.x;
-            g.y += v.y;
+#pragma unroll
+            for (int p = 0; p < NC; ++p) {
+                const double2 v = Gbuf[((int64_t)slot * NC + p) * estride + env];
+                g[p].x += v.x;
+                g[p].y += v.y;
+            }
         }
     }
-    return g;
 }
 
 template <int L, int PROP>
@@ -64,9 +70,11 @@ __global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
     double2* Gs = reinterpret_cast<double2*>(smem);
     double* Rs = reinterpret_cast<double*>(Gs + npool * NC);  // [64][2*nr]
     if (env >= nenv) return;
-    for (int q = lane; q < npool * NC; q += 64) {
-        const int k = q / NC, p = q % NC;
-        Gs[q] = gather_g<NC>(gslot_ptr, gslot_list, gcls, cls_u, Gbuf, estride, env, k, p);
+    for (int k = lane; k < npool; k += 64) {
+        double2 g[NC];
+        gather_g<NC>(gslot_ptr, gslot_list, gcls, cls_u, Gbuf, estride, env, k, g);
+#pragma unroll
+        for (int p = 0; p < NC; ++p) Gs[k * NC + p] = g[p];
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -202,8 +210,10 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
     for (int k = lane; k < npool; k += 64) {
         const int mu = max(0, pool[4 * k]), n = pool[4 * k + 1] - 1, l = pool[4 * k + 2], m = pool[4 * k + 3];
         const int d = ((mu * NP + idxp(l, m)) * nr + n) * NC;
+        double2 g[NC];
+        gather_g<NC>(gslot_ptr, gslot_list, gcls, cls_u, Gbuf, estride, env, k, g);
 #pragma unroll
-        for (int p = 0; p < NC; ++p) Gd[d + p] = gather_g<NC>(gslot_ptr, gslot_list, gcls, cls_u, Gbuf, estride, env, k, p);
+        for (int p = 0; p < NC; ++p) Gd[d + p] = g[p];
     }
     __syncthreads();
     const int j0 = off[env], j1 = off[env + 1];
