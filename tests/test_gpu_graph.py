@@ -11,16 +11,26 @@ from helpers import basis_for
 pytestmark = pytest.mark.gpu
 
 
-def test_graph_replays_match_eager(cuda):
-    print("[gpu] graph capture of energy+forces, order-4 degree-12 basis, 300 envs", flush=True)
-    basis, T = basis_for(4, 12, 6)
+GRAPH_CASES = [
+    ("order4-degree12", (4, 12, 6), {}, 2, 300, 0),
+    # class-split EuclideanVector, 32-environment record tiles, 2 parts per
+    # tile (the class shares summed by k_esum)
+    ("config4-small", (3, 10, 5), {"prop": "EuclideanVector", "species": 2}, 4, 150, 2),
+]
+
+
+@pytest.mark.parametrize("name,args,kw,wid,nenv,parts", GRAPH_CASES, ids=[c[0] for c in GRAPH_CASES])
+def test_graph_replays_match_eager(cuda, name, args, kw, wid, nenv, parts):
+    print(f"[gpu] graph capture of energy+forces, {name}, {nenv} envs", flush=True)
+    basis, T = basis_for(*args, **kw)
     c = synth.coefficients(len(basis), 2)
-    off, R, _ = synth.workload_envs(2, nenv=300)
-    off2, R2, _ = synth.workload_envs(2, nenv=90, seed=77)
+    off, R, sp = synth.workload_envs(wid, nenv=nenv)
+    off2, R2, sp2 = synth.workload_envs(wid, nenv=90, seed=77)
     ev = HIPEvaluator(basis, c, device=0, waves=16)
+    ev.set_parts(parts)
     ev.set_validation(False)  # host-side checks copy to the host: not capturable
-    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
-    b2 = EnvBatch(off2, R2.T.copy(), None, device="cuda:0")
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    b2 = EnvBatch(off2, R2.T.copy(), sp2, device="cuda:0")
     E0, dE0 = ev.energy_forces(b)  # eager reference; sizes every buffer
     torch.cuda.synchronize()
     E = torch.empty_like(E0)
