@@ -217,7 +217,7 @@ class HIPEvaluator:
 
     def basis_dB(self, b: EnvBatch, stream=None):
         shape = (b.nnbr, self.nB, 3) if self.prop == 0 else (b.nnbr, self.nB, 3, 3)
-        out = torch.zeros(shape, dtype=torch.float64, device=b.xyz.device)
+        out = torch.empty(shape, dtype=torch.float64, device=b.xyz.device)  # every entry is written
         _lib.check(self.lib.ace_basis_dB(self.ctx, b.nenv, _vp(b.offsets), _vp(b.xyz), _vp(b.species),
                                          _vp(out), self._stream(stream)))
         return out

@@ -333,6 +333,170 @@ __global__ void k_dB(int64_t nnbr, int nB, int nAA, const int32_t* __restrict__ 
         for (int d = 0; d < 3; ++d) dB[(i * NC + p) * 3 + d] = s[p][d];
 }
 
+// k_dBw: per-neighbour basis gradients through per-environment adjoint
+// weights (symmbasis.jl:322-336 with pibasis.jl:402-432 reorganised).
+// dB[j][b] = Re sum_k W+[b][k] dphi+[j][k], where W+ collects, for every A2B
+// entry z of row b and factor t of its AA column, the A2B value times the
+// product of the other factors (prefix and suffix products) times the
+// factor's sign, conjugated for conjugated (m < 0) factors, in the slot of
+// the factor's pooled function.  One workgroup per (environment, chunk of up
+// to 256 basis functions); thread = basis function.  W+ of the chunk lives in
+// LDS; the neighbours' pooled gradients dphi+ [npool][3] are staged DB_NJ at a
+// time (as many as DB_STAGE double2 hold), the next stage loaded into
+// registers while the current one is used.  The weights are built once per
+// workgroup, so a (environment, chunk) pair costs O(A2B entries x order) for
+// the weights and O(neighbours x slots) for the gradients, instead of the
+// dAA[neighbour][AA function] the materialising path writes.
+struct DbArgs {
+    const int32_t* chunk_b;   // [nchunk + 1]
+    const int32_t* chunk_s;   // [nchunk + 1]
+    const int32_t* b_slot;    // [nB + 1]
+    const int32_t* slot_k;    // [nslots]
+    const uint4* ztab;        // [nnz][MO / 4]: factor and slot codes per A2B entry
+    const int32_t* rowptr;
+    const double* vals;       // [nnz] or [nnz][6]
+    int npool, nB;
+};
+
+constexpr int DB_NJ = DB_NJ_MAX;  // neighbours per stage (at most)
+constexpr int DB_STAGE = 2048;    // double2 per stage (8 per thread)
+
+template <int PROP, int MO>
+__global__ __launch_bounds__(256) void k_dBw(DbArgs a, int64_t nenv, const int32_t* __restrict__ off,
+                                             const double2* __restrict__ Aplus, int64_t estride,
+                                             const double2* __restrict__ dphi, double* __restrict__ dB) {
+    constexpr int NC = PROP == 0 ? 1 : 3;
+    extern __shared__ double2 sm[];
+    const int64_t env = blockIdx.x;
+    const int c = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int b0 = a.chunk_b[c], b1 = a.chunk_b[c + 1], s0 = a.chunk_s[c], s1 = a.chunk_s[c + 1];
+    const int ns = s1 - s0;
+    const int nq = 3 * a.npool;                           // double2 per neighbour
+    const int njs = max(1, min(DB_NJ, DB_STAGE / nq));     // neighbours per stage
+    double2* As = sm;                       // [npool]
+    double2* dp = As + a.npool;             // [njs][npool][3]
+    double2* W = dp + (size_t)njs * nq;     // [ns][NC]
+    int* sk = reinterpret_cast<int*>(W + (size_t)ns * NC);  // [ns]
+    for (int k = tid; k < a.npool; k += 256) As[k] = Aplus[(int64_t)k * estride + env];
+    for (int q = tid; q < ns; q += 256) {
+        sk[q] = a.slot_k[s0 + q];
+#pragma unroll
+        for (int p = 0; p < NC; ++p) W[q * NC + p] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int b = b0 + tid;
+    const bool active = b < b1;
+    const int sb = active ? a.b_slot[b] - s0 : 0;
+    const int nsb = active ? a.b_slot[b + 1] - a.b_slot[b] : 0;
+    if (active) {
+        for (int z = a.rowptr[b]; z < a.rowptr[b + 1]; ++z) {
+            uint32_t code[MO];
+#pragma unroll
+            for (int h = 0; h < MO / 4; ++h) {
+                const uint4 w4 = a.ztab[(size_t)z * (MO / 4) + h];
+                code[4 * h] = w4.x;
+                code[4 * h + 1] = w4.y;
+                code[4 * h + 2] = w4.z;
+                code[4 * h + 3] = w4.w;
+            }
+            if (!(code[0] & (1u << 13))) continue;  // order 0: no gradient
+            double2 f[MO];
+            double sg[MO];
+#pragma unroll
+            for (int q = 0; q < MO; ++q) {
+                f[q] = make_double2(1.0, 0.0);
+                sg[q] = 1.0;
+                if (code[q] & (1u << 13)) {
+                    double2 v = As[code[q] & 0x3ffu];
+                    if (code[q] & 0x8000u) v.y = -v.y;
+                    sg[q] = (code[q] & 0x4000u) ? -1.0 : 1.0;
+                    f[q] = make_double2(sg[q] * v.x, sg[q] * v.y);
+                }
+            }
+            double2 pre[MO];
+            pre[0] = make_double2(1.0, 0.0);
+#pragma unroll
+            for (int q = 1; q < MO; ++q) pre[q] = cmul(pre[q - 1], f[q - 1]);
+            double2 suf = make_double2(1.0, 0.0);
+#pragma unroll
+            for (int t = MO - 1; t >= 0; --t) {
+                if (code[t] & (1u << 13)) {
+                    const double2 P = cmul(pre[t], suf);
+                    const uint32_t sc = code[t] >> 16;
+                    const bool cj = (sc & 0x8000u) != 0;
+                    double2* w = W + (size_t)(sb + (int)(sc & 0x7fffu)) * NC;
+#pragma unroll
+                    for (int p = 0; p < NC; ++p) {
+                        double2 v;
+                        if (PROP == 0) {
+                            const double cv = a.vals[z] * sg[t];
+                            v = make_double2(cv * P.x, cv * P.y);
+                        } else {
+                            const double2 cv = make_double2(a.vals[6 * z + 2 * p] * sg[t], a.vals[6 * z + 2 * p + 1] * sg[t]);
+                            v = cmul(cv, P);
+                        }
+                        w[p].x += v.x;
+                        w[p].y += cj ? -v.y : v.y;
+                    }
+                    suf = cmul(suf, f[t]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int j0 = off[env], j1 = off[env + 1];
+    // stages of njs neighbours: their dphi+ rows are consecutive in memory
+    double2 pf[DB_STAGE / 256];
+    auto fetch = [&](int jb) {
+        const int n = min(njs, j1 - jb) * nq;
+        const double2* src = dphi + (int64_t)jb * nq;
+#pragma unroll
+        for (int r = 0; r < DB_STAGE / 256; ++r) {
+            const int q = tid + 256 * r;
+            if (q < n) pf[r] = src[q];
+        }
+    };
+    if (j0 < j1) fetch(j0);
+    for (int jb = j0; jb < j1; jb += njs) {
+        const int nj = min(njs, j1 - jb);
+#pragma unroll
+        for (int r = 0; r < DB_STAGE / 256; ++r) {
+            const int q = tid + 256 * r;
+            if (q < nj * nq) dp[q] = pf[r];
+        }
+        __syncthreads();
+        if (jb + njs < j1) fetch(jb + njs);  // in flight while this stage is used
+        if (active) {
+            for (int jj = 0; jj < nj; ++jj) {
+                const double2* d = dp + (size_t)jj * nq;
+                double acc[NC][3];
+#pragma unroll
+                for (int p = 0; p < NC; ++p) acc[p][0] = acc[p][1] = acc[p][2] = 0.0;
+                for (int q = 0; q < nsb; ++q) {
+                    const int k = sk[sb + q];
+                    const double2 x0 = d[3 * k], x1 = d[3 * k + 1], x2 = d[3 * k + 2];
+#pragma unroll
+                    for (int p = 0; p < NC; ++p) {
+                        const double2 w = W[(size_t)(sb + q) * NC + p];
+                        acc[p][0] += w.x * x0.x - w.y * x0.y;
+                        acc[p][1] += w.x * x1.x - w.y * x1.y;
+                        acc[p][2] += w.x * x2.x - w.y * x2.y;
+                    }
+                }
+                double* out = dB + ((int64_t)(jb + jj) * a.nB + b) * NC * 3;
+#pragma unroll
+                for (int p = 0; p < NC; ++p) {
+                    out[3 * p] = acc[p][0];
+                    out[3 * p + 1] = acc[p][1];
+                    out[3 * p + 2] = acc[p][2];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_nbr_env(int64_t nenv, const int32_t* __restrict__ off, int32_t* __restrict__ out) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nenv) return;
@@ -469,6 +633,74 @@ static void launch_dphi(ace_ctx* ctx, hipStream_t st, int64_t nnbr, const double
                        ctx->nA, ctx->dphi.p);
 }
 
+template <int L>
+static void launch_dphi_pool(ace_ctx* ctx, hipStream_t st, int64_t nnbr, const double* xyz,
+                             const int8_t* species) {
+    // the pooled (m >= 0) functions: pool rows are (mu, n, l, m) like aspec rows
+    hipLaunchKernelGGL(k_dphi<L>, dim3((unsigned)((nnbr + 63) / 64)), dim3(64), 0, st, radial_params(ctx),
+                       nnbr, ctx->nbr_env.p, xyz, ctx->nspecies > 0 ? species : nullptr, ctx->pool.p,
+                       ctx->npool, ctx->dphi.p);
+}
+
+// Basis gradients through per-environment adjoint weights (k_dBw): 0, or 1
+// when the tables do not fit the kernel's LDS (the caller then runs the
+// materialising path).
+static int basis_dB_weights(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* off, const double* xyz,
+                            const int8_t* species, int64_t nnbr, double* dB_out) {
+    if (!ctx->db_usable) return 1;
+    const int NC = ctx->prop == ACE_PROP_INVARIANT ? 1 : 3;
+    const size_t fixed = db_fixed_lds(ctx->npool);
+    const size_t per_slot = (size_t)NC * sizeof(double2) + sizeof(int32_t);
+    if (!ctx->db_ready) {
+        acehost::DbTables& t = ctx->db_host;
+        // no empty uploads (a basis without A2B entries)
+        if (t.slot_k.empty()) t.slot_k.push_back(0);
+        if (t.ztab.size() < 4) t.ztab.resize(4, 0u);
+        if (ctx->db_chunk_b.upload(t.chunk_b.data(), t.chunk_b.size()) ||
+            ctx->db_chunk_s.upload(t.chunk_s.data(), t.chunk_s.size()) ||
+            ctx->db_b_slot.upload(t.b_slot.data(), t.b_slot.size()) ||
+            ctx->db_slot_k.upload(t.slot_k.data(), t.slot_k.size()) ||
+            ctx->db_ztab.upload(t.ztab.data(), t.ztab.size()))
+            return -1;
+        ctx->db_nchunk = (int)t.chunk_b.size() - 1;
+        ctx->db_max_slots = t.max_chunk_slots;
+        ctx->db_ready = true;
+    }
+    const int64_t estride = ((nenv + 63) / 64) * 64;
+    if (ctx->Aplus.ensure((size_t)ctx->npool * estride) || ctx->nbr_env.ensure(nnbr) ||
+        ctx->dphi.ensure((size_t)nnbr * ctx->npool * 3))
+        return -1;
+    if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), false)) return -1;
+    hipLaunchKernelGGL(k_nbr_env, dim3((unsigned)((nenv + 255) / 256)), dim3(256), 0, st, nenv, off,
+                       ctx->nbr_env.p);
+    DISPATCH_L(ctx->L, launch_dphi_pool, ctx, st, nnbr, xyz, species);
+    HIP_TRY(hipGetLastError());
+    DbArgs a;
+    a.chunk_b = ctx->db_chunk_b.p;
+    a.chunk_s = ctx->db_chunk_s.p;
+    a.b_slot = ctx->db_b_slot.p;
+    a.slot_k = ctx->db_slot_k.p;
+    a.ztab = reinterpret_cast<const uint4*>(ctx->db_ztab.p);
+    a.rowptr = ctx->rowptr.p;
+    a.vals = ctx->vals.p;
+    a.npool = ctx->npool;
+    a.nB = ctx->nB;
+    if (ctx->db_nchunk == 0) return 0;  // no basis functions
+    const size_t lds = fixed + (size_t)ctx->db_max_slots * per_slot;
+    const dim3 grid((unsigned)nenv, (unsigned)ctx->db_nchunk);
+    auto go = [&](auto kern) {
+        set_func_lds((const void*)kern, lds);
+        hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, a, nenv, off, ctx->Aplus.p, estride, ctx->dphi.p, dB_out);
+    };
+    const bool wide = ctx->db_host.zw == 8;
+    if (ctx->prop == ACE_PROP_INVARIANT)
+        wide ? go(k_dBw<0, 8>) : go(k_dBw<0, 4>);
+    else
+        wide ? go(k_dBw<1, 8>) : go(k_dBw<1, 4>);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 extern "C" int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
                             const int8_t* species, double* dB_out, void* stream) {
     int64_t nnbr = 0;
@@ -476,6 +708,9 @@ extern "C" int ace_basis_dB(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
     if (validate_inputs(ctx, nenv, off, nnbr, species, (hipStream_t)stream)) return -1;
     if (nenv == 0 || nnbr == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
+    const int rc = basis_dB_weights(ctx, st, nenv, off, xyz, species, nnbr, dB_out);
+    if (rc <= 0) return rc;
+    // bases whose tables exceed the weight kernel's LDS: dAA per neighbour
     if (basis_A_internal(ctx, nenv, off, xyz, species, nnbr, st)) return -1;
     if (ctx->nbr_env.ensure(nnbr) || ctx->dphi.ensure((size_t)nnbr * ctx->nA * 3) ||
         ctx->dAA.ensure((size_t)nnbr * ctx->nAA * 3))

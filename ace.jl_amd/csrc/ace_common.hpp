@@ -148,6 +148,19 @@ __device__ __forceinline__ int64_t xcd_env(int64_t b, int64_t n) {
 }
 
 constexpr int POOL_CH = 16;
+// LDS of the basis-gradient weight kernel (k_dBw), and neighbours staged per
+// barrier: measured at config 2, one neighbour per stage within 48 KB (three
+// workgroups per CU) beat 2 or 4 neighbours per stage (fewer functions per
+// chunk, fewer workgroups per CU) and a 32 KB budget
+constexpr int DB_NJ_MAX = 1;
+constexpr size_t DB_LDS_BUDGET = 48 * 1024;
+// k_dBw's LDS besides the weights: A+ [npool] and a stage of up to DB_NJ_MAX
+// neighbours' dphi+ [npool][3] within 2048 double2 (at least one neighbour)
+inline size_t db_fixed_lds(int npool) {
+    const size_t nq = 3 * (size_t)npool;
+    const size_t njs = std::max<size_t>(1, std::min<size_t>(DB_NJ_MAX, 2048 / nq));
+    return ((size_t)npool + njs * nq) * sizeof(double2);
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -233,6 +246,14 @@ struct ace_ctx {
     // pooling launch shape (ace_set_pool_shape): neighbours per chunk (8, 16,
     // 32; 0 = 16) and environments per wave (0 = automatic, 1 = one)
     int pool_ch = 0, pool_epw = 0;
+    // basis-gradient weight tables: built on the host at context creation,
+    // uploaded at the first ace_basis_dB call; db_usable: they fit the
+    // weight kernel's LDS budget (DB_LDS_BUDGET)
+    acehost::DbTables db_host;
+    bool db_usable = false, db_ready = false;
+    int db_nchunk = 0, db_max_slots = 0;
+    DevBuf<int32_t> db_chunk_b, db_chunk_s, db_b_slot, db_slot_k;
+    DevBuf<uint32_t> db_ztab;
     DevBuf<double2> cls_u;        // [3 classes][3 components]
     DevBuf<int32_t> gslot_cls;    // [ngslots] class of every G slot
     acehost::PackCopy host;

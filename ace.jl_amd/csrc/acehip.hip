@@ -744,6 +744,17 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
                 return fail("upload");
         }
     }
+    {
+        // tables of the basis-gradient weight kernel (ace_basis_dB)
+        const int NC = pk->prop == ACE_PROP_INVARIANT ? 1 : 3;
+        const size_t fixed = db_fixed_lds(pk->npool);
+        const size_t per_slot = (size_t)NC * sizeof(double2) + sizeof(int32_t);
+        if (fixed + 64 * per_slot <= DB_LDS_BUDGET) {
+            const int max_slots = (int)((DB_LDS_BUDGET - fixed) / per_slot);
+            ctx->db_host = acehost::db_tables(pk, 256, max_slots);
+            ctx->db_usable = ctx->db_host.max_chunk_slots <= max_slots;
+        }
+    }
     for (auto& t : wt) {
         WaveCfg* wc = new WaveCfg();
         ctx->wcfg.push_back(wc);

@@ -491,6 +491,71 @@ inline void lpt_lists(const PackCopy& p, const std::vector<int>& tasks, int V, s
     }
 }
 
+// Tables of the per-environment basis-gradient kernel: basis function b's
+// gradient is Re sum_k W+[b][k] dphi+[j][k] over the pooled (m >= 0)
+// functions k it depends on, with W+ built per environment from the A2B
+// entries of row b.  For entry z (AA column col, factors iA_t) and factor t,
+// the product of the other factors times the A2B value goes to the slot of
+// pooled function k(iA_t) in b's slot list, conjugated when iA_t is a
+// conjugated (m < 0) function.
+//   b_slot[nB + 1]      CSR of slots per basis function
+//   slot_k[nslots]      pooled function of a slot
+//   ztab[nnz][zw]       per A2B entry, one word per factor t < zw (zw = 4
+//                       for correlation order <= 4, else 8): low half the
+//                       factor (pooled k in bits 0-9, bit 13 present, bit 14
+//                       negative sign, bit 15 conjugated), high half the slot
+//                       index within b's list (bits 0-14) | conjugated << 15
+//   chunk_b[nchunk + 1], chunk_s[nchunk + 1]: basis-function chunks of at
+//   most max_b functions and max_slots slots (one workgroup each)
+struct DbTables {
+    std::vector<int32_t> b_slot, slot_k, chunk_b, chunk_s;
+    std::vector<uint32_t> ztab;
+    int zw = 4;
+    int max_chunk_slots = 0;
+};
+
+inline DbTables db_tables(const ace_pack* pk, int max_b, int max_slots) {
+    DbTables t;
+    const int mo = pk->maxorder;
+    t.zw = mo <= 4 ? 4 : 8;
+    t.b_slot.assign((size_t)pk->nB + 1, 0);
+    t.ztab.assign((size_t)pk->a2b_nnz * t.zw, 0u);
+    std::vector<int> where((size_t)pk->npool, -1);
+    for (int b = 0; b < pk->nB; ++b) {
+        const int s0 = (int)t.slot_k.size();
+        for (int z = pk->a2b_rowptr[b]; z < pk->a2b_rowptr[b + 1]; ++z) {
+            const int col = pk->a2b_cols[z];
+            for (int q = 0; q < pk->orders[col]; ++q) {
+                const int iA = pk->iAA2iA[(size_t)col * mo + q];
+                const int f = pk->aspec_fid[iA];
+                const int k = f & 0x7fff;
+                if (where[(size_t)k] < 0) {
+                    where[(size_t)k] = (int)t.slot_k.size() - s0;
+                    t.slot_k.push_back(k);
+                }
+                const uint32_t cj = (f & 0x8000) ? 1u : 0u;
+                const uint32_t fc = (uint32_t)k | (1u << 13) | (pk->aspec_sgn[iA] < 0 ? 1u << 14 : 0u) | (cj << 15);
+                const uint32_t sc = (uint32_t)where[(size_t)k] | (cj << 15);
+                t.ztab[(size_t)z * t.zw + q] = fc | (sc << 16);
+            }
+        }
+        for (size_t i = (size_t)s0; i < t.slot_k.size(); ++i) where[(size_t)t.slot_k[i]] = -1;
+        t.b_slot[(size_t)b + 1] = (int32_t)t.slot_k.size();
+    }
+    t.chunk_b.push_back(0);
+    t.chunk_s.push_back(0);
+    for (int b = 0; b < pk->nB;) {
+        int e = b;
+        while (e < pk->nB && e - b < max_b && t.b_slot[(size_t)e + 1] - t.b_slot[(size_t)b] <= max_slots) ++e;
+        if (e == b) e = b + 1;  // one function above the slot budget: a chunk of its own
+        t.max_chunk_slots = std::max(t.max_chunk_slots, t.b_slot[(size_t)e] - t.b_slot[(size_t)b]);
+        t.chunk_b.push_back(e);
+        t.chunk_s.push_back(t.b_slot[(size_t)e]);
+        b = e;
+    }
+    return t;
+}
+
 // (mu, l, m, n) -> pooled index, -1 when absent (force kernel lookup).
 inline std::vector<int16_t> pool_lookup(const ace_pack* pk) {
     const int ns = std::max(1, (int)pk->nspecies);

@@ -404,3 +404,30 @@ def test_pool_shapes_bit_identical(cuda, args, wid, nenv):
         assert torch.equal(E, E0) and torch.equal(dE, dE0), (ch, epw)
     n = 16
     assert rel_err(E0.cpu().numpy()[:n], O.energy(T, c, off[: n + 1], R[: off[n]])) < E_TOL
+
+
+DB_CASES = [
+    ("config2", (4, 12, 6), {}, 3),                                            # 3235 B: 13 chunks of 256
+    ("two-species", (3, 6, 3), {"species": 2}, 6),
+    ("order5", (5, 6, 3), {}, 6),                                              # products of up to 5 factors
+    ("config4", (3, 10, 5), {"prop": "EuclideanVector", "species": 2}, 2),     # 3 components per slot
+]
+
+
+@pytest.mark.parametrize("name,args,kw,nenv", DB_CASES, ids=[c[0] for c in DB_CASES])
+def test_basis_dB_weight_kernel(cuda, name, args, kw, nenv):
+    """ace_basis_dB through per-environment adjoint weights (one workgroup per
+    environment and chunk of basis functions) against the oracle's dB =
+    A2B . dAA (per neighbour, all basis functions, all directions)."""
+    print(f"[gpu] basis dB weight kernel: {name}", flush=True)
+    basis, T = basis_for(*args, **kw)
+    c = synth.coefficients(len(basis), 3)
+    ns = kw.get("species", 0)
+    off, R, sp = random_batch(nenv, 60 + nenv, nmin=3, nmax=25, nspecies=ns)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    dB = ev.basis_dB(b).cpu().numpy()
+    _, dAA = O.eval_AA_dAA(T, off, R, sp)
+    ref = O.eval_dB(T, dAA)
+    assert dB.size == ref.size
+    assert rel_err(dB.reshape(ref.shape), ref) < F_TOL
