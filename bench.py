@@ -27,9 +27,9 @@ FP64_PEAK_TFLOPS = 78.6
 # tiles split over 3 workgroups, 4 for the tiles of the last round), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the
 # factors a 256 MiB device copy gives the same counters (2.0 for FETCH_SIZE,
-# 1.0 for WRITE_SIZE): 714.9 MB read + 269.7 MB written.  Only valid for the
+# 1.0 for WRITE_SIZE): 720.8 MB read + 269.7 MB written.  Only valid for the
 # default --nenv; re-measured whenever the kernel changes.
-DAG_TRAFFIC_BYTES = {100_000: 984.5e6}
+DAG_TRAFFIC_BYTES = {100_000: 990.4e6}
 WORKLOAD_NAMES = {
     1: "plain ACE energy+forces, 1000 envs x 12 neighbours, order 3, degree 6, Lmax 4, Invariant, FP64",
     2: "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], order 4, degree 12, Lmax 6, "
