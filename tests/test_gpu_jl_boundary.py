@@ -232,3 +232,30 @@ def test_lmax9_basis(cuda):
     torch.cuda.synchronize()
     assert rel_err(E.cpu().numpy(), O.energy(T, c, off, R)) < E_TOL
     assert rel_err(dE.cpu().numpy().T, O.grad_config(T, c, off, R)) < F_TOL
+
+
+@pytest.mark.parametrize("prop", ["Invariant", "EuclideanVector"])
+def test_three_species_beyond_320_pooled_values(cuda, prop):
+    """Three species, order 2, degree 12, L 6: 417 (Invariant) and 498
+    (EuclideanVector, class-split) pooled values, beyond the 32-environment
+    record tiles: the run interpreter on 16-environment tiles (four task
+    streams per wave).  50 environments are four tiles; every energy and
+    gradient against the oracle, energy-only calls included, both compilers."""
+    print(f"[gpu] three species, npool > 320, {prop}", flush=True)
+    basis, T = basis_for(2, 12, 6, species=3, prop=prop)
+    c = synth.coefficients(len(basis), 35)
+    off, R, sp = random_batch(50, 36, nspecies=3)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    E_ref = O.energy(T, c, off, R, sp)
+    g_ref = O.grad_config(T, c, off, R, sp)
+    for compiler in ("native", "python"):
+        ev = HIPEvaluator(basis, c, device=0, waves=8, compiler=compiler)
+        E, dE = ev.energy_forces(b)
+        E1 = ev.energy(b)
+        torch.cuda.synchronize()
+        assert ev.last_dag_shape()[2] == 16
+        E, dE, E1 = E.cpu().numpy(), dE.cpu().numpy(), E1.cpu().numpy()
+        assert rel_err(E, E_ref) < E_TOL
+        assert rel_err(E1, E_ref) < E_TOL
+        g = dE.T if prop == "Invariant" else dE.reshape(3, 3, -1).transpose(2, 0, 1)
+        assert rel_err(g, g_ref) < F_TOL
