@@ -135,8 +135,9 @@ class HIPEvaluator:
         _lib.check(self.lib.ace_set_pool_shape(self.ctx, int(chunk), int(envs_per_wave)))
 
     def set_kernel(self, mode: int = 0):
-        """Product-DAG kernel: 0 the record kernel where the basis allows it
-        (default), 1 the run interpreter always."""
+        """Product-DAG kernel: 0 the record-segment kernel where the basis
+        allows it (default), 1 the run interpreter always, 2 the per-record
+        dispatch kernel over the same records (comparison)."""
         _lib.check(self.lib.ace_set_kernel(self.ctx, int(mode)))
 
     def set_tail_parts(self, ntail: int = -1):

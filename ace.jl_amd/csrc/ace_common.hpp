@@ -243,6 +243,9 @@ struct ace_ctx {
     // force_interp: run the interpreter even so (ace_set_kernel)
     int rec_et = 0;
     bool force_interp = false;
+    // rec_dispatch: the record kernel that switches per record (ace_set_kernel
+    // mode 2) instead of the segment kernel
+    bool rec_dispatch = false;
     // pooling launch shape (ace_set_pool_shape): neighbours per chunk (8, 16,
     // 32; 0 = 16) and environments per wave (0 = automatic, 1 = one)
     int pool_ch = 0, pool_epw = 0;
@@ -276,6 +279,8 @@ struct ace_ctx {
     DevBuf<double> leaf_coef, task_const, e0v;
     DevBuf<double4> vleaf;  // EuclideanVector decoded leaves (two per leaf; one when class-split)
     DevBuf<uint32_t> recs;
+    DevBuf<uint32_t> segw, segt;  // segment kernel: record words, segments [kind, count, offset, 0]
+    DevBuf<int32_t> seg_ptr;      // [ntasks + 1] segments of every task
     // surrogate feature tables
     int nfeat = 0;
     std::vector<int32_t> feat_rows;

@@ -228,6 +228,10 @@ struct DagRecArgs {
     int64_t ntile_lo;
     const int32_t* wtask_ptr_hi;  // [(parts + 1) * W + 1]
     const int32_t* wtask_list_hi;
+    // segment kernel (null: the per-record kernel)
+    const uint32_t* segw;
+    const uint4* segt;
+    const int32_t* seg_ptr;
 };
 
 // Per-lane choice between two record words (the half-wave mode), taken by
@@ -239,11 +243,21 @@ __device__ __forceinline__ double rec_coef(uint32_t lo, uint32_t hi) {
     return __hiloint2double((int)hi, (int)lo);
 }
 
-// c * v for a complex coefficient held as a uint4 (re lo, re hi, im lo, im hi)
+// S (+)= c * v for a complex coefficient held as a uint4 (re lo, re hi, im
+// lo, im hi), as one chain of fused multiply-adds per component (FIRST: S
+// starts from this product)
+template <bool FIRST>
 __device__ __forceinline__ void rec_cfma(double2& S, const uint4& c, const double2& v) {
     const double cr = rec_coef(c.x, c.y), ci = rec_coef(c.z, c.w);
-    S.x += cr * v.x - ci * v.y;
-    S.y += cr * v.y + ci * v.x;
+    if (FIRST) {
+        S.x = cr * v.x;
+        S.y = cr * v.y;
+    } else {
+        S.x = fma(cr, v.x, S.x);
+        S.y = fma(cr, v.y, S.y);
+    }
+    S.x = fma(-ci, v.y, S.x);
+    S.y = fma(ci, v.x, S.y);
 }
 
 // One record of a given type (MORE: leaves 2 and 3 present; U1 / U2 / U3:
@@ -272,25 +286,27 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
     if (U1) a1 = at(h1.y);
     if (U2) a2 = at(h1.z);
     if (U3) a3 = at(h1.w);
-    double2 S = make_double2(0.0, 0.0);
+    // S = sum of the leaves' coefficient times value, accumulated in one
+    // fused multiply-add chain per component
+    double2 S;
     if (CPLX) {
         if (HALF) {
             // per-lane choice of the coefficient words (field by field: a
             // whole-record select would go through memory)
             const uint4 c0 = make_uint4(pick(hi, h2.x, h3.x), pick(hi, h2.y, h3.y), pick(hi, h2.z, h3.z),
                                         pick(hi, h2.w, h3.w));
-            rec_cfma(S, c0, v0);
+            rec_cfma<true>(S, c0, v0);
             if (MORE) {
                 const uint4 c1 = make_uint4(pick(hi, h4.x, h5.x), pick(hi, h4.y, h5.y), pick(hi, h4.z, h5.z),
                                             pick(hi, h4.w, h5.w));
-                rec_cfma(S, c1, v1);
+                rec_cfma<false>(S, c1, v1);
             }
         } else {
-            rec_cfma(S, h2, v0);
-            rec_cfma(S, h3, v1);
+            rec_cfma<true>(S, h2, v0);
+            rec_cfma<false>(S, h3, v1);
             if (MORE) {
-                rec_cfma(S, h4, v2);
-                rec_cfma(S, h5, v3);
+                rec_cfma<false>(S, h4, v2);
+                rec_cfma<false>(S, h5, v3);
             }
         }
     } else if (HALF) {
@@ -299,17 +315,17 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
         S.y = c0 * v0.y;
         if (MORE) {
             const double c1 = rec_coef(pick(hi, h3.x, h3.z), pick(hi, h3.y, h3.w));
-            S.x += c1 * v1.x;
-            S.y += c1 * v1.y;
+            S.x = fma(c1, v1.x, S.x);
+            S.y = fma(c1, v1.y, S.y);
         }
     } else {
         const double c0 = rec_coef(h2.x, h2.y), c1 = rec_coef(h2.z, h2.w);
-        S.x = c0 * v0.x + c1 * v1.x;
-        S.y = c0 * v0.y + c1 * v1.y;
+        S.x = fma(c1, v1.x, c0 * v0.x);
+        S.y = fma(c1, v1.y, c0 * v0.y);
         if (MORE) {
             const double c2 = rec_coef(h3.x, h3.y), c3 = rec_coef(h3.z, h3.w);
-            S.x += c2 * v2.x + c3 * v3.x;
-            S.y += c2 * v2.y + c3 * v3.y;
+            S.x = fma(c3, v3.x, fma(c2, v2.x, S.x));
+            S.y = fma(c3, v3.y, fma(c2, v2.y, S.y));
         }
     }
     S.y = flip_sign_if(S.y, ctl >> 16);  // bit 31, conjugated leaves: conj(sum c A) = sum c conj(A)
@@ -327,8 +343,17 @@ __device__ __forceinline__ void dagr_record(const char* lds, uint32_t ctl, const
         a3.y = flip_sign_if(a3.y, ctl >> 15);  // bit 30
         Pc = cmul(P2, a3);
     }
-    acc.x += Pc.x * S.x - Pc.y * S.y;
-    acc.y += Pc.x * S.y + Pc.y * S.x;
+    acc.x = fma(-Pc.y, S.y, fma(Pc.x, S.x, acc.x));
+    acc.y = fma(Pc.y, S.x, fma(Pc.x, S.y, acc.y));
+}
+
+// Record words through the constant address space: the address is
+// wave-uniform, so every fetch is a scalar load and the coefficients and LDS
+// offsets are scalar operands without lane-to-scalar copies.
+typedef unsigned int RecWords __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 rec_load(const uint4* p) {
+    const RecWords v = *(const __attribute__((address_space(4))) RecWords*)(uintptr_t)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 template <bool CPLX, bool HALF>
@@ -386,10 +411,10 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __rest
         // (1 for the prefix-free runs, which come first).
         double2 P1 = make_double2(1.0, 0.0), P2 = P1, Pc = P1;
         const uint4* rp = recs + RQ * (int64_t)rb;
-        uint4 h0 = rp[0], h1 = rp[1], h2 = rp[2], h3 = rp[3], h4, h5;
+        uint4 h0 = rec_load(rp), h1 = rec_load(rp + 1), h2 = rec_load(rp + 2), h3 = rec_load(rp + 3), h4, h5;
         if (CPLX) {
-            h4 = rp[4];
-            h5 = rp[5];
+            h4 = rec_load(rp + 4);
+            h5 = rec_load(rp + 5);
         }
         // unrolled by 2: the prefetched and current record headers alternate
         // between two register sets instead of being copied every record
@@ -398,11 +423,11 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __rest
         for (int r = rb; r < re; ++r) {
             // next record (record nrec exists and is all zeros)
             const uint4* np = recs + RQ * (int64_t)(r + 1);
-            const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            const uint4 n0 = rec_load(np), n1 = rec_load(np + 1), n2 = rec_load(np + 2), n3 = rec_load(np + 3);
             uint4 n4, n5;
             if (CPLX) {
-                n4 = np[4];
-                n5 = np[5];
+                n4 = rec_load(np + 4);
+                n5 = rec_load(np + 5);
             }
             const uint32_t ctl = h0.x;
 #define ACE_REC(T)                                                                                          \
@@ -476,6 +501,225 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __rest
             }
         } else {
             E_out[env] = Q[0];  // e0 + the waves' sums, in that order
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_dags: the record programs regrouped into kind-sorted segments of
+// self-contained records.  Every record recomputes its whole prefix product,
+// so the records of a task can be reordered: each task is a short list of
+// segments, each a run of records of one kind (prefix depth D = 0..3, NL =
+// 1..4 leaf slots), and the kernel switches once per segment, not once per
+// record.  The loop over a segment has no branches, so the LDS reads of
+// consecutive records overlap.  Record words (flags, prefix offsets, leaf
+// offsets, padding to an even count, coefficients):
+//   w[0]              bit 31: the leaves are conjugated; bit 31 - t: prefix
+//                     factor t (1..3) is conjugated
+//   w[1 .. D]         LDS byte offsets of the prefix factors
+//   w[1 + D ..]       LDS byte offsets of the NL leaves (32-environment
+//                     tiles: NL = 2 or 4, the low half-wave takes the even
+//                     leaf slots and the high half the odd ones)
+//   then NL FP64 coefficients (CPLX: NL complex), zero for padding slots.
+template <int D, int NL, bool CPLX>
+struct SegShape {
+    static constexpr int HW = (2 + D + NL) & ~1;           // header words, even
+    static constexpr int RW = HW + NL * (CPLX ? 4 : 2);     // record words
+};
+
+typedef const __attribute__((address_space(4))) uint32_t SegWord;
+
+__device__ __forceinline__ double xor_sign(double c, uint32_t m) {
+    // negate c when bit 31 of m is set
+    return __hiloint2double(__double2hiint(c) ^ (int)(m & 0x80000000u), __double2loint(c));
+}
+
+template <int D, int NL, bool CPLX, bool HALF>
+__device__ __forceinline__ void dags_segment(const char* lds, const uint32_t* wp, int count, bool hi,
+                                             double2& acc) {
+    using SH = SegShape<D, NL, CPLX>;
+    constexpr int HW = SH::HW, RW = SH::RW;
+    constexpr int NV = HALF ? NL / 2 : NL;  // leaves per lane
+    auto at = [&](uint32_t off) { return *reinterpret_cast<const double2*>(lds + off); };
+    const SegWord* w = (const SegWord*)(uintptr_t)wp;
+#pragma unroll 2
+    for (int i = 0; i < count; ++i, w += RW) {
+        uint32_t h[RW];
+#pragma unroll
+        for (int q = 0; q < RW; ++q) h[q] = w[q];
+        double2 v[NV], f[D > 0 ? D : 1];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = at(HALF ? pick(hi, h[1 + D + 2 * q], h[2 + D + 2 * q]) : h[1 + D + q]);
+#pragma unroll
+        for (int t = 0; t < D; ++t) f[t] = at(h[1 + t]);
+        // S = sum of coefficient times leaf value: one fused multiply-add
+        // chain per component
+        double2 S;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int z = HALF ? 2 * q : q;  // leaf slot (HALF: the low half's)
+            if (CPLX) {
+                const uint4 c = HALF ? make_uint4(pick(hi, h[HW + 4 * z], h[HW + 4 * z + 4]),
+                                                  pick(hi, h[HW + 4 * z + 1], h[HW + 4 * z + 5]),
+                                                  pick(hi, h[HW + 4 * z + 2], h[HW + 4 * z + 6]),
+                                                  pick(hi, h[HW + 4 * z + 3], h[HW + 4 * z + 7]))
+                                     : make_uint4(h[HW + 4 * z], h[HW + 4 * z + 1], h[HW + 4 * z + 2],
+                                                  h[HW + 4 * z + 3]);
+                if (q == 0)
+                    rec_cfma<true>(S, c, v[q]);
+                else
+                    rec_cfma<false>(S, c, v[q]);
+            } else {
+                const double c = HALF ? rec_coef(pick(hi, h[HW + 2 * z], h[HW + 2 * z + 2]),
+                                                 pick(hi, h[HW + 2 * z + 1], h[HW + 2 * z + 3]))
+                                      : rec_coef(h[HW + 2 * z], h[HW + 2 * z + 1]);
+                if (q == 0) {
+                    S.x = c * v[q].x;
+                    S.y = c * v[q].y;
+                } else {
+                    S.x = fma(c, v[q].x, S.x);
+                    S.y = fma(c, v[q].y, S.y);
+                }
+            }
+        }
+        S.y = xor_sign(S.y, h[0]);
+        if (D == 0) {
+            acc.x += S.x;
+            acc.y += S.y;
+        } else {
+            double2 P = f[0];
+            P.y = xor_sign(P.y, h[0] << 1);
+#pragma unroll
+            for (int t = 1; t < D; ++t) {
+                double2 g = f[t];
+                g.y = xor_sign(g.y, h[0] << (t + 1));
+                P = cmul(P, g);
+            }
+            acc.x = fma(-P.y, S.y, fma(P.x, S.x, acc.x));
+            acc.y = fma(P.y, S.x, fma(P.x, S.y, acc.y));
+        }
+    }
+}
+
+template <bool CPLX, bool HALF>
+__global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __restrict__ segw,
+                                               const uint4* __restrict__ segt, const int32_t* __restrict__ seg_ptr,
+                                               int64_t nenv, const double2* __restrict__ Aplus, int64_t estride,
+                                               double* __restrict__ E_out, double2* __restrict__ Gbuf,
+                                               const double* __restrict__ e0p, const int32_t* __restrict__ env_list,
+                                               const int32_t* __restrict__ env_count) {
+    constexpr int ET = HALF ? 32 : 64;     // environments per tile
+    constexpr int NE = CPLX ? 6 : 1;       // energy partials per lane
+    extern __shared__ double2 As[];  // [npool][ET]
+    const int lane = threadIdx.x & 63;
+    const int el = HALF ? (lane & 31) : lane;
+    const bool hi = HALF && lane >= 32;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t blo = a.ntile_lo * a.parts;
+    const bool tail = (int64_t)blockIdx.x >= blo;
+    const int P = tail ? a.parts + 1 : a.parts;
+    const int64_t bt = tail ? (int64_t)blockIdx.x - blo : (int64_t)blockIdx.x;
+    const int64_t tile = (tail ? a.ntile_lo : 0) + bt / P;
+    const int part = (int)(bt % P);
+    const int32_t* const wptr = tail ? a.wtask_ptr_hi : a.wtask_ptr;
+    const int32_t* const wlist = tail ? a.wtask_list_hi : a.wtask_list;
+    int64_t env = tile * ET + el;
+    bool valid = env < nenv;
+    if (env_list) {
+        const int64_t cnt = *env_count;
+        if (tile * ET >= cnt) return;  // whole workgroup idle: before any barrier
+        valid = env < cnt;
+        env = valid ? env_list[env] : 0;
+    }
+    {
+        const int k0 = HALF ? 2 * wave + (hi ? 1 : 0) : wave, kstep = HALF ? 2 * a.W : a.W;
+        for (int k = k0; k < a.npool; k += kstep)
+            As[k * ET + el] = valid ? nt_load2(Aplus + (int64_t)k * estride + env) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+
+    const char* lds = reinterpret_cast<const char*>(As) + 16 * el;
+    double ep[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) ep[q] = 0.0;
+    const SegWord* sp = (const SegWord*)(uintptr_t)seg_ptr;
+    const int tb = wptr[part * a.W + wave], te = wptr[part * a.W + wave + 1];
+    for (int ti = tb; ti < te; ++ti) {
+        const int t = wlist[ti];
+        const int slot = a.tasks[5 * t + 2];
+        double2 acc = (HALF && hi) ? make_double2(0.0, 0.0) : make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
+        const int sb = (int)sp[t], se = (int)sp[t + 1];
+        for (int sg = sb; sg < se; ++sg) {
+            const SegWord* st = (const SegWord*)(uintptr_t)(segt + sg);
+            const uint32_t kind = st[0], count = st[1], off = st[2];
+            const uint32_t* wp = segw + off;
+#define ACE_SEG(DD, NN)                                                                  \
+    case DD * 4 + NN - 1:                                                                \
+        dags_segment<DD, NN, CPLX, HALF>(lds, wp, (int)count, hi, acc);                  \
+        break;
+            if constexpr (HALF) {
+                switch (kind) { ACE_SEG(0, 2) ACE_SEG(0, 4) ACE_SEG(1, 2) ACE_SEG(1, 4)
+                                ACE_SEG(2, 2) ACE_SEG(2, 4) ACE_SEG(3, 2) ACE_SEG(3, 4) }
+            } else {
+                switch (kind) {
+                    ACE_SEG(0, 1) ACE_SEG(0, 2) ACE_SEG(0, 3) ACE_SEG(0, 4)
+                    ACE_SEG(1, 1) ACE_SEG(1, 2) ACE_SEG(1, 3) ACE_SEG(1, 4)
+                    ACE_SEG(2, 1) ACE_SEG(2, 2) ACE_SEG(2, 3) ACE_SEG(2, 4)
+                    ACE_SEG(3, 1) ACE_SEG(3, 2) ACE_SEG(3, 3) ACE_SEG(3, 4)
+                }
+            }
+#undef ACE_SEG
+        }
+        if (HALF) {
+            acc.x += __shfl_xor(acc.x, 32);
+            acc.y += __shfl_xor(acc.y, 32);
+        }
+        if (slot < 0) {
+            if (CPLX) {
+                const int c = -1 - slot;
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    if (c == q) {
+                        ep[2 * q] += acc.x;
+                        ep[2 * q + 1] += acc.y;
+                    }
+            } else {
+                ep[0] += acc.x;
+            }
+        } else if (valid && !hi) {
+            nt_store2(Gbuf + (int64_t)slot * estride + env, acc);
+        }
+    }
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(As);  // [W][NE][ET]
+    if (!hi) {
+#pragma unroll
+        for (int q = 0; q < NE; ++q) red[(wave * NE + q) * ET + el] = ep[q];
+    }
+    __syncthreads();
+    if (wave == 0 && !hi && valid && E_out) {
+        double Q[NE];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            Q[q] = (!CPLX && a.parts == 1) ? *e0p : 0.0;
+            for (int w = 0; w < a.W; ++w) Q[q] += red[(w * NE + q) * ET + el];
+        }
+        if (a.parts > 1) {
+#pragma unroll
+            for (int q = 0; q < NE; ++q) E_out[((int64_t)part * NE + q) * estride + env] = Q[q];
+        } else if (CPLX) {
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                double sum = e0p[p];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 u = a.cls_u[3 * c + p];
+                    sum += u.x * Q[2 * c] - u.y * Q[2 * c + 1];
+                }
+                E_out[env * 3 + p] = sum;
+            }
+        } else {
+            E_out[env] = Q[0];
         }
     }
 }
@@ -555,6 +799,12 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
         const std::string rerr = acehost::records(ctx->host, cf, rec, ctx->rec_et == 64 ? 10 : 9);
         if (!rerr.empty()) return set_err(rerr);
         if (ctx->recs.upload(rec.data(), rec.size())) return -1;
+        acehost::Segments sg;
+        const std::string serr = acehost::segments(ctx->host, cf, ctx->rec_et == 64 ? 10 : 9, ctx->rec_et == 32, sg);
+        if (!serr.empty()) return set_err(serr);
+        if (ctx->segw.upload(sg.words.data(), sg.words.size()) || ctx->segt.upload(sg.segt.data(), sg.segt.size()) ||
+            ctx->seg_ptr.upload(sg.seg_ptr.data(), sg.seg_ptr.size()))
+            return -1;
     }
     if (ctx->task_const.upload(cf.task.data(), cf.task.size())) return -1;
     double ev[3] = {cf.e0[0], cf.e0[2], cf.e0[4]};
@@ -916,8 +1166,10 @@ extern "C" int ace_set_pool_shape(ace_ctx* ctx, int chunk, int envs_per_wave) {
 
 extern "C" int ace_set_kernel(ace_ctx* ctx, int mode) {
     if (!ctx) return set_err("ace_set_kernel: null context");
-    if (mode != 0 && mode != 1) return set_err("ace_set_kernel: mode must be 0 (automatic) or 1 (run interpreter)");
+    if (mode < 0 || mode > 2)
+        return set_err("ace_set_kernel: mode must be 0 (automatic), 1 (run interpreter) or 2 (per-record dispatch)");
     ctx->force_interp = mode == 1;
+    ctx->rec_dispatch = mode == 2;
     return 0;
 }
 
@@ -1037,12 +1289,23 @@ static int launch_dag_interp(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int
                 : launch_dag_prop<1, 3>(ctx, st, a, nenv, estride, ws, E_out);
 }
 
-// k_dagr for a tile of ET (64 or 32) environments.
+// The record kernel for a tile of ET (64 or 32) environments: the segment
+// kernel k_dags when ra carries segments, else k_dagr.
 template <bool CPLX>
 static void launch_dagr(int ET, dim3 grid, dim3 block, size_t lds, hipStream_t st, const DagRecArgs& ra,
                         int64_t nenv, const double2* Aplus, int64_t estride, double* E_out, double2* Gbuf,
                         const double* e0p, const int32_t* env_list, const int32_t* env_count) {
-    if (ET == 64) {
+    if (ra.segw) {
+        if (ET == 64) {
+            set_func_lds((const void*)k_dags<CPLX, false>, lds);
+            hipLaunchKernelGGL((k_dags<CPLX, false>), grid, block, lds, st, ra, ra.segw, ra.segt, ra.seg_ptr, nenv,
+                               Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
+        } else {
+            set_func_lds((const void*)k_dags<CPLX, true>, lds);
+            hipLaunchKernelGGL((k_dags<CPLX, true>), grid, block, lds, st, ra, ra.segw, ra.segt, ra.seg_ptr, nenv,
+                               Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
+        }
+    } else if (ET == 64) {
         set_func_lds((const void*)k_dagr<CPLX, false>, lds);
         hipLaunchKernelGGL((k_dagr<CPLX, false>), grid, block, lds, st, ra, reinterpret_cast<const uint4*>(ra.recs),
                            nenv, Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
@@ -1130,6 +1393,9 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
         ctx->last_tail = nhi;
         DagRecArgs ra;
         ra.recs = ctx->recs.p;
+        ra.segw = ctx->rec_dispatch ? nullptr : ctx->segw.p;
+        ra.segt = reinterpret_cast<const uint4*>(ctx->segt.p);
+        ra.seg_ptr = ctx->seg_ptr.p;
         ra.tasks = ctx->d_tasks.p;
         ra.task_const = ctx->task_const.p;
         ra.wtask_ptr = P == 1 ? a.wtask_ptr : (forces ? wc->pp_all[P].p : wc->pp_e[P].p);
@@ -1253,6 +1519,9 @@ int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estrid
     const int64_t ntile = (nenv + 63) / 64;
     DagRecArgs ra;
     ra.recs = ctx->recs.p;
+    ra.segw = ctx->rec_dispatch ? nullptr : ctx->segw.p;
+    ra.segt = reinterpret_cast<const uint4*>(ctx->segt.p);
+    ra.seg_ptr = ctx->seg_ptr.p;
     ra.tasks = ctx->d_tasks.p;
     ra.task_const = ctx->task_const.p;
     ra.wtask_ptr = forces ? wc->d_ptr_all.p : wc->d_ptr_e.p;

@@ -460,6 +460,83 @@ inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint
     return "";
 }
 
+// The same records as kind-sorted segments of self-contained records (the
+// segment kernel): per task, its records grouped by kind = depth * 4 + leaf
+// slots - 1 (stable within a kind), one segment per kind present.  Record
+// words: flags (bit 31 conjugated leaves, bit 31 - t conjugated prefix factor
+// t), the depth prefix offsets, the leaf slot offsets, padding to an even
+// count, then the slot coefficients (FP64, or complex for class-split
+// programs).  half: 32-environment tiles, leaf slots rounded up to 2 or 4
+// (the two half-waves take alternate slots); padding slots repeat leaf 0's
+// offset with coefficient 0.
+//   segt[4 * s]     kind, record count, word offset of the first record, 0
+//   seg_ptr[t]      the segments of task t: [seg_ptr[t], seg_ptr[t + 1])
+struct Segments {
+    std::vector<uint32_t> words, segt;
+    std::vector<int32_t> seg_ptr;
+};
+
+inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, bool half, Segments& out) {
+    out.words.clear();
+    out.segt.clear();
+    out.seg_ptr.assign(1, 0);
+    auto off = [&](uint32_t f) { return (f & 0x7fffu) << lds_shift; };
+    const int CW = p.classed ? 4 : 2;  // coefficient words per slot
+    for (int t = 0; t < p.ntasks; ++t) {
+        const int rb = p.tasks[5 * (size_t)t + 3], re = p.tasks[5 * (size_t)t + 4];
+        std::vector<std::vector<int>> by_kind(16);
+        for (int r = rb; r < re; ++r) {
+            const uint32_t* h = p.rec_hdr.data() + 8 * (size_t)r;
+            const int d = h[0] & 3, nl = (h[0] >> 5) & 7;
+            if (nl < 1 || nl > 4) return "record without leaves";
+            const int ns = half ? (nl <= 2 ? 2 : 4) : nl;
+            by_kind[d * 4 + ns - 1].push_back(r);
+        }
+        for (int kind = 0; kind < 16; ++kind) {
+            const std::vector<int>& rs = by_kind[kind];
+            if (rs.empty()) continue;
+            const int d = kind / 4, ns = kind % 4 + 1;
+            const int hw = (2 + d + ns) & ~1, rw = hw + ns * CW;
+            out.segt.insert(out.segt.end(), {(uint32_t)kind, (uint32_t)rs.size(), (uint32_t)out.words.size(), 0u});
+            for (int r : rs) {
+                const uint32_t* h = p.rec_hdr.data() + 8 * (size_t)r;
+                const int nl = (h[0] >> 5) & 7;
+                const uint32_t lid[4] = {h[3] & 0xffffu, h[3] >> 16, h[4] & 0xffffu, h[4] >> 16};
+                const uint32_t fid[3] = {h[1] & 0xffffu, h[1] >> 16, h[2] & 0xffffu};
+                const bool cj = (lid[0] & 0x8000u) != 0;
+                for (int i = 0; i < nl; ++i)
+                    if (((lid[i] & 0x8000u) != 0) != cj) return "record leaves mix conjugation";
+                std::vector<uint32_t> w((size_t)rw, 0u);
+                uint32_t fl = cj ? 0x80000000u : 0u;
+                for (int q = 0; q < d; ++q) {
+                    if (fid[q] & 0x8000u) fl |= 0x80000000u >> (q + 1);
+                    w[1 + q] = off(fid[q]);
+                }
+                w[0] = fl;
+                for (int i = 0; i < ns; ++i) w[1 + d + i] = off(lid[i < nl ? i : 0]);
+                for (int i = 0; i < nl; ++i) {
+                    const int lf = p.rec_leaf[4 * (size_t)r + i];
+                    if (lf < 0) return "record leaf slot without a leaf";
+                    double c[2];
+                    if (p.classed) {
+                        c[0] = cf.leaf[2 * (size_t)lf];
+                        c[1] = cj ? -cf.leaf[2 * (size_t)lf + 1] : cf.leaf[2 * (size_t)lf + 1];
+                    } else {
+                        c[0] = cf.leaf[lf];
+                    }
+                    std::memcpy(w.data() + hw + (size_t)CW * i, c, sizeof(double) * (CW / 2));
+                }
+                out.words.insert(out.words.end(), w.begin(), w.end());
+            }
+        }
+        out.seg_ptr.push_back((int32_t)(out.segt.size() / 4));
+    }
+    // slack after the last record (the kernel never reads past a segment)
+    out.words.resize(out.words.size() + 64, 0u);
+    if (out.segt.empty()) out.segt.assign(4, 0u);
+    return "";
+}
+
 // Task lists of the two kernel modes: every task (energy + forces), energy only.
 inline std::vector<int> mode_tasks(const PackCopy& p, bool forces) {
     std::vector<int> out;

@@ -23,7 +23,7 @@ sys.path.insert(0, os.path.join(ROOT, "ace.jl_amd"))
 METRIC = "env/s energy+forces: plain FP64 vs surrogate at ≤1 meV/atom MAE"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per k_dagr launch on this workload (100k environments, 16-wave
+# HBM bytes per product-DAG launch on this workload (100k environments, 16-wave
 # tiles split over 3 workgroups, 4 for the tiles of the last round), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the
 # factors a 256 MiB device copy gives the same counters (2.0 for FETCH_SIZE,
@@ -66,22 +66,41 @@ def algorithmic_flops(pack, forces=True):
     return fe + float(np.sum(6 * c + 4 * o2))
 
 
-def executed_flops(pack, forces=True):
-    """FLOPs the product-DAG kernel executes per environment: 4 per leaf
-    (real x complex accumulate; 8 for the complex coefficients of class-split
-    EuclideanVector programs), 6 per prefix product, 8 per run (complex
-    multiply-accumulate of prefix x leaf sum)."""
-    w = pack.work_counts(forces)
-    per_leaf = 8.0 if getattr(pack, "classed", 0) else 4.0
-    return per_leaf * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
+def executed_flops(pack, forces=True, kernel=0):
+    """FLOPs the product-DAG kernel executes per environment.
+
+    Segment kernel (kernel code 0 / 1: 64- / 32-environment tiles), per record
+    of depth d with s leaf slots: 4 per slot, less 2 for the first (a
+    multiply), 8 for the complex coefficients of class-split EuclideanVector
+    programs (less 4); the whole prefix product, 6 per factor after the first;
+    8 for the accumulation (2 without a prefix).  On 32-environment tiles the
+    slots are rounded up to 2 or 4 and both half-waves of an environment
+    compute the prefix and the accumulation.  Run interpreter: 4 per leaf, 6
+    per new prefix product, 8 per run."""
+    import numpy as np
+    cplx = bool(getattr(pack, "classed", 0))
+    if kernel not in (0, 1):
+        w = pack.work_counts(forces)
+        per_leaf = 8.0 if cplx else 4.0
+        return per_leaf * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
+    tasks = list(pack.energy_tasks) + (list(pack.grad_tasks) if forces else [])
+    recs = np.concatenate([np.arange(pack.tasks[t][3], pack.tasks[t][4]) for t in tasks]) if tasks else []
+    h = pack.rec_hdr[recs, 0].astype(np.int64)
+    d, nl = h & 3, (h >> 5) & 7
+    lanes = 2 if kernel == 1 else 1
+    slots = np.where(nl <= 2, 2, 4) if kernel == 1 else nl
+    per_slot = 8.0 if cplx else 4.0
+    leaf = per_slot * slots - per_slot / 2 * lanes
+    pre = 6.0 * np.maximum(d - 1, 0) + np.where(d > 0, 8.0, 2.0)
+    return float(np.sum(leaf + lanes * pre))
 
 
 def dag_kernel_name(code):
     """The product-DAG kernel of ace_last_dag_shape's code."""
     if code == 0:
-        return "k_dagr (64-environment tiles)"
+        return "k_dags (kind-sorted record segments, 64-environment tiles)"
     if code == 1:
-        return "k_dagr (32-environment tiles, two half-waves per record)"
+        return "k_dags (kind-sorted record segments, 32-environment tiles, two half-waves per record)"
     return f"k_dag run interpreter ({code}-environment tiles)"
 
 
@@ -335,7 +354,7 @@ def main():
     pk = ev.pack
     dag_parts, dag_tail, dag_kernel = ev.last_dag_shape()
     f_alg = algorithmic_flops(pk, True)
-    f_exe = executed_flops(pk, True)
+    f_exe = executed_flops(pk, True, dag_kernel)
     dag_avg = float(np.mean(dag_ms))
     achieved = f_alg * b.nenv / (dag_avg * 1e-3) / 1e12
     achieved_exe = f_exe * b.nenv / (dag_avg * 1e-3) / 1e12

@@ -237,11 +237,12 @@ int ace_set_tail_parts(ace_ctx* ctx, int ntail);
  * Instrumentation; no counterpart in the reference. */
 int ace_last_dag_shape(ace_ctx* ctx, int* parts, int64_t* tail_tiles, int* kernel);
 
-/* Product-DAG kernel choice: 0 (default) the record kernel where the basis
- * allows it (Invariant or class-split EuclideanVector programs, correlation
- * order <= 4, at most 320 pooled values), else the run interpreter; 1 the
- * run interpreter always.  A tuning and testing knob of this implementation;
- * no counterpart in the reference. */
+/* Product-DAG kernel choice: 0 (default) the record-segment kernel where the
+ * basis allows it (Invariant or class-split EuclideanVector programs,
+ * correlation order <= 4, at most 320 pooled values), else the run
+ * interpreter; 1 the run interpreter always; 2 the per-record dispatch kernel
+ * over the same records where they exist.  A tuning and testing knob of this
+ * implementation; no counterpart in the reference. */
 int ace_set_kernel(ace_ctx* ctx, int mode);
 
 /* Pooling launch shape: neighbours staged per chunk (8, 16 or 32; 0 = 16)

@@ -88,7 +88,9 @@ def test_invariant_above_160_pooled_values(cuda, deg, order, nenv):
     """One species, degree 14, L 6: 195 pooled values, too many for a
     64-environment LDS tile.  The record kernel runs 32-environment tiles (the
     two half-waves on one record stream, two leaves each); the run interpreter
-    (forced) runs 32-environment tiles with two task streams per wave."""
+    (forced) runs 32-environment tiles with two task streams per wave; mode 2
+    runs the records with per-record dispatch instead of kind-sorted
+    segments."""
     print(f"[gpu] npool 195 invariant, order {order}, degree {deg}", flush=True)
     basis, T = basis_for(order, deg, 6)
     c = synth.coefficients(len(basis), 12)
@@ -98,7 +100,7 @@ def test_invariant_above_160_pooled_values(cuda, deg, order, nenv):
     E_ref = O.energy(T, c, off, R)
     n = 40 if order == 4 else nenv
     g_ref = O.grad_config(T, c, off[: n + 1], R[: off[n]])
-    for mode, kern in ((0, 1), (1, 32)):
+    for mode, kern in ((0, 1), (1, 32), (2, 1)):
         ev.set_kernel(mode)
         E, dE = ev.energy_forces(b)
         E1 = ev.energy(b)

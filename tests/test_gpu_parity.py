@@ -59,6 +59,28 @@ def test_config2_tile_parts(cuda, parts):
             assert rel_err(dE.T, O.grad_config(T, c, off, R)) < F_TOL
 
 
+@pytest.mark.parametrize("args,wid,nenv,kern", [((4, 12, 6), 2, 130, 0)], ids=["config2-64env"])
+def test_segment_and_per_record_kernels(cuda, args, wid, nenv, kern):
+    """The kind-sorted segment kernel (default) and the per-record dispatch
+    kernel (mode 2) run the same records in different orders: both match the
+    oracle (64-environment tiles, 139 pooled values; the 32-environment tiles
+    are covered by the 195-pooled-value test)."""
+    print(f"[gpu] segment vs per-record kernel, basis {args}, {nenv} envs", flush=True)
+    basis, T = basis_for(*args)
+    c = synth.coefficients(len(basis), 2)
+    off, R, sp = synth.workload_envs(wid, nenv=nenv)
+    ev = HIPEvaluator(basis, c, device=0, waves=16)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    Eref, gref = O.energy(T, c, off, R), O.grad_config(T, c, off, R)
+    for mode in (0, 2):
+        ev.set_kernel(mode)
+        E, dE = ev.energy_forces(b)
+        torch.cuda.synchronize()
+        assert ev.last_dag_shape()[2] == kern
+        assert rel_err(E.cpu().numpy(), Eref) < E_TOL
+        assert rel_err(dE.cpu().numpy().T, gref) < F_TOL
+
+
 @pytest.mark.parametrize("deg", [6, 12])
 def test_force_blocks_of_short_environments(cuda, deg):
     """The flat force kernel stages the G of a 256-neighbour block's
@@ -223,14 +245,16 @@ def test_config3_fcc_forces_fd(cuda):
 _VEC_REF = {}
 
 
-@pytest.mark.parametrize("mode,parts,tail", [(0, 0, -1), (0, 2, -1), (0, 3, 1), (1, 0, -1)],
-                         ids=["records", "records-2parts", "records-3parts-tail", "interpreter"])
+@pytest.mark.parametrize("mode,parts,tail", [(0, 0, -1), (0, 2, -1), (0, 3, 1), (1, 0, -1), (2, 2, -1)],
+                         ids=["records", "records-2parts", "records-3parts-tail", "interpreter",
+                              "per-record-2parts"])
 def test_vector_property(cuda, mode, parts, tail):
     """EuclideanVector, order 3, degree 6 (class-split programs): 64-environment
     tiles; 150 environments are two full tiles and a partial one.  The record
-    kernel with complex leaf coefficients, with the tiles split over 2 and 3
-    workgroups (and one tail tile of 4), and the run interpreter; energies
-    with and without forces."""
+    kernel with complex leaf coefficients (kind-sorted segments; mode 2 the
+    per-record dispatch kernel), with the tiles split over 2 and 3 workgroups
+    (and one tail tile of 4), and the run interpreter; energies with and
+    without forces."""
     print(f"[gpu] EuclideanVector order 3 degree 6, 150 envs, kernel mode {mode}, parts {parts}", flush=True)
     basis, T = basis_for(3, 6, 4, prop="EuclideanVector")
     c = synth.coefficients(len(basis), 4)
@@ -243,8 +267,8 @@ def test_vector_property(cuda, mode, parts, tail):
     E, dE = ev.energy_forces(b)
     torch.cuda.synchronize()
     P, ntail, kern = ev.last_dag_shape()
-    assert kern == (0 if mode == 0 else 64)
-    if mode == 0 and parts:
+    assert kern == (64 if mode == 1 else 0)
+    if mode != 1 and parts:
         assert P == parts and ntail == max(0, tail)
     E1 = ev.energy(b)
     E, dE, E1 = E.cpu().numpy(), dE.cpu().numpy(), E1.cpu().numpy()
@@ -355,12 +379,13 @@ def test_invalid_inputs_raise(cuda):
 _C4_REF = {}
 
 
-@pytest.mark.parametrize("waves,mode", [(8, 0), (16, 0), (8, 1)], ids=["records-w8", "records-w16", "interpreter-w8"])
+@pytest.mark.parametrize("waves,mode", [(8, 0), (16, 0), (8, 1), (8, 2)],
+                         ids=["records-w8", "records-w16", "interpreter-w8", "per-record-w8"])
 def test_config4_two_species_vector(cuda, waves, mode):
     """Config 4 (2 species, EuclideanVector, order 3, degree 10, L 5): 220
     pooled values, 32-environment tiles: the record kernel with the two
-    half-waves on one record stream, or the run interpreter with two task
-    streams per wave.  100 environments are three full tiles and a partial
+    half-waves on one record stream (kind-sorted segments; mode 2 per-record
+    dispatch), or the run interpreter with two task streams per wave.  100 environments are three full tiles and a partial
     one; every energy and every per-neighbour Jacobian is compared with the
     oracle."""
     print(f"[gpu] config4: 100 envs, waves={waves}, kernel mode {mode} (32-env tiles)", flush=True)
@@ -372,7 +397,7 @@ def test_config4_two_species_vector(cuda, waves, mode):
     b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
     E, dE = ev.energy_forces(b)
     torch.cuda.synchronize()
-    assert ev.last_dag_shape()[2] == (1 if mode == 0 else 32)
+    assert ev.last_dag_shape()[2] == (32 if mode == 1 else 1)
     E, dE = E.cpu().numpy(), dE.cpu().numpy()
     if not _C4_REF:
         _C4_REF["E"] = O.energy(T, c, off, R, sp)
