@@ -232,6 +232,7 @@ struct DagRecArgs {
     const uint32_t* segw;
     const uint4* segt;
     const int32_t* seg_ptr;
+    bool deep;  // prefixes of more than 3 factors (correlation order > 4)
 };
 
 // Per-lane choice between two record words (the half-wave mode), taken by
@@ -542,7 +543,8 @@ __device__ __forceinline__ void dags_segment(const char* lds, const uint32_t* wp
     constexpr int NV = HALF ? NL / 2 : NL;  // leaves per lane
     auto at = [&](uint32_t off) { return *reinterpret_cast<const double2*>(lds + off); };
     const SegWord* w = (const SegWord*)(uintptr_t)wp;
-#pragma unroll 2
+    // unrolled by 3 (real coefficients) or 2 (complex): measured fastest
+#pragma unroll(CPLX ? 2 : 3)
     for (int i = 0; i < count; ++i, w += RW) {
         uint32_t h[RW];
 #pragma unroll
@@ -601,7 +603,7 @@ __device__ __forceinline__ void dags_segment(const char* lds, const uint32_t* wp
     }
 }
 
-template <bool CPLX, bool HALF>
+template <bool CPLX, bool HALF, bool DEEP>
 __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __restrict__ segw,
                                                const uint4* __restrict__ segt, const int32_t* __restrict__ seg_ptr,
                                                int64_t nenv, const double2* __restrict__ Aplus, int64_t estride,
@@ -657,9 +659,23 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
     case DD * 4 + NN - 1:                                                                \
         dags_segment<DD, NN, CPLX, HALF>(lds, wp, (int)count, hi, acc);                  \
         break;
+            // DEEP: prefixes of 4 to 7 factors (correlation orders 5 to 8),
+            // 64-environment tiles only
+            static_assert(!(HALF && DEEP), "deep prefixes run on 64-environment tiles");
             if constexpr (HALF) {
                 switch (kind) { ACE_SEG(0, 2) ACE_SEG(0, 4) ACE_SEG(1, 2) ACE_SEG(1, 4)
                                 ACE_SEG(2, 2) ACE_SEG(2, 4) ACE_SEG(3, 2) ACE_SEG(3, 4) }
+            } else if constexpr (DEEP) {
+                switch (kind) {
+                    ACE_SEG(0, 1) ACE_SEG(0, 2) ACE_SEG(0, 3) ACE_SEG(0, 4)
+                    ACE_SEG(1, 1) ACE_SEG(1, 2) ACE_SEG(1, 3) ACE_SEG(1, 4)
+                    ACE_SEG(2, 1) ACE_SEG(2, 2) ACE_SEG(2, 3) ACE_SEG(2, 4)
+                    ACE_SEG(3, 1) ACE_SEG(3, 2) ACE_SEG(3, 3) ACE_SEG(3, 4)
+                    ACE_SEG(4, 1) ACE_SEG(4, 2) ACE_SEG(4, 3) ACE_SEG(4, 4)
+                    ACE_SEG(5, 1) ACE_SEG(5, 2) ACE_SEG(5, 3) ACE_SEG(5, 4)
+                    ACE_SEG(6, 1) ACE_SEG(6, 2) ACE_SEG(6, 3) ACE_SEG(6, 4)
+                    ACE_SEG(7, 1) ACE_SEG(7, 2) ACE_SEG(7, 3) ACE_SEG(7, 4)
+                }
             } else {
                 switch (kind) {
                     ACE_SEG(0, 1) ACE_SEG(0, 2) ACE_SEG(0, 3) ACE_SEG(0, 4)
@@ -794,11 +810,13 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
         }
         if (ctx->vleaf.upload(vl.data(), vl.size())) return -1;
     }
-    if (ctx->rec_et > 0) {
+    if (ctx->rec_et > 0 && ctx->nrec > 0) {
         std::vector<uint32_t> rec;
         const std::string rerr = acehost::records(ctx->host, cf, rec, ctx->rec_et == 64 ? 10 : 9);
         if (!rerr.empty()) return set_err(rerr);
         if (ctx->recs.upload(rec.data(), rec.size())) return -1;
+    }
+    if (ctx->rec_et > 0) {
         acehost::Segments sg;
         const std::string serr = acehost::segments(ctx->host, cf, ctx->rec_et == 64 ? 10 : 9, ctx->rec_et == 32, sg);
         if (!serr.empty()) return set_err(serr);
@@ -877,12 +895,13 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
     ctx->nrec = pk->nrec;
     ctx->ncoef = pk->prop == ACE_PROP_INVARIANT ? 1 : 6;
     ctx->classed = pk->classed;
-    // the record kernel's tile: 64 environments while their pooled values fit
-    // LDS (16 B each: npool <= 160), else 32 (npool <= 320)
-    if ((pk->prop == ACE_PROP_INVARIANT || pk->classed) && pk->nrec > 0)
-        ctx->rec_et = (size_t)pk->npool * 64 * sizeof(double2) <= 160 * 1024   ? 64
-                      : (size_t)pk->npool * 32 * sizeof(double2) <= 160 * 1024 ? 32
-                                                                               : 0;
+    // the record kernels' tile: 64 environments while their pooled values fit
+    // LDS (16 B each: npool <= 160), else 32 (npool <= 320; correlation order
+    // <= 4 only, deeper bases of that size run the interpreter)
+    if (pk->prop == ACE_PROP_INVARIANT || pk->classed)
+        ctx->rec_et = (size_t)pk->npool * 64 * sizeof(double2) <= 160 * 1024 ? 64
+                      : (size_t)pk->npool * 32 * sizeof(double2) <= 160 * 1024 && pk->maxorder <= 4 ? 32
+                                                                                                     : 0;
     ctx->host = acehost::copy_pack(pk);
     const int nc = ctx->ncoef;
     auto fail = [&](const char* what) {
@@ -1296,15 +1315,20 @@ static void launch_dagr(int ET, dim3 grid, dim3 block, size_t lds, hipStream_t s
                         int64_t nenv, const double2* Aplus, int64_t estride, double* E_out, double2* Gbuf,
                         const double* e0p, const int32_t* env_list, const int32_t* env_count) {
     if (ra.segw) {
-        if (ET == 64) {
-            set_func_lds((const void*)k_dags<CPLX, false>, lds);
-            hipLaunchKernelGGL((k_dags<CPLX, false>), grid, block, lds, st, ra, ra.segw, ra.segt, ra.seg_ptr, nenv,
-                               Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
-        } else {
-            set_func_lds((const void*)k_dags<CPLX, true>, lds);
-            hipLaunchKernelGGL((k_dags<CPLX, true>), grid, block, lds, st, ra, ra.segw, ra.segt, ra.seg_ptr, nenv,
-                               Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
-        }
+#define ACE_DAGS(HALF_, DEEP_)                                                                              \
+    do {                                                                                                    \
+        set_func_lds((const void*)k_dags<CPLX, HALF_, DEEP_>, lds);                                         \
+        hipLaunchKernelGGL((k_dags<CPLX, HALF_, DEEP_>), grid, block, lds, st, ra, ra.segw, ra.segt,        \
+                           ra.seg_ptr, nenv, Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);        \
+    } while (0)
+        // deep prefixes only on 64-environment tiles (the context's choice)
+        if (ET == 64 && !ra.deep)
+            ACE_DAGS(false, false);
+        else if (ET == 64)
+            ACE_DAGS(false, true);
+        else
+            ACE_DAGS(true, false);
+#undef ACE_DAGS
     } else if (ET == 64) {
         set_func_lds((const void*)k_dagr<CPLX, false>, lds);
         hipLaunchKernelGGL((k_dagr<CPLX, false>), grid, block, lds, st, ra, reinterpret_cast<const uint4*>(ra.recs),
@@ -1346,6 +1370,8 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
     // order <= 4 with at most 320 pooled values; everything else runs the run
     // interpreter
     const bool use_records = ctx->rec_et > 0 && !ctx->force_interp;
+    if (use_records && ctx->rec_dispatch && ctx->nrec == 0)
+        return set_err("ace_set_kernel mode 2: per-record dispatch needs correlation order <= 4");
     ctx->last_tail = 0;
     if (use_records) {
         const int ET = ctx->rec_et;
@@ -1396,6 +1422,7 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
         ra.segw = ctx->rec_dispatch ? nullptr : ctx->segw.p;
         ra.segt = reinterpret_cast<const uint4*>(ctx->segt.p);
         ra.seg_ptr = ctx->seg_ptr.p;
+        ra.deep = ctx->maxorder > 4;
         ra.tasks = ctx->d_tasks.p;
         ra.task_const = ctx->task_const.p;
         ra.wtask_ptr = P == 1 ? a.wtask_ptr : (forces ? wc->pp_all[P].p : wc->pp_e[P].p);
@@ -1512,8 +1539,9 @@ extern "C" int ace_last_timing(ace_ctx* ctx, double* dag_ms, double* pool_ms, do
 
 int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estride, bool forces, double* E_out,
                       const int32_t* env_list, const int32_t* env_count) {
-    if (ctx->prop != ACE_PROP_INVARIANT || ctx->nrec == 0)
-        return set_err("listed product-DAG evaluation: Invariant bases of correlation order <= 4 only");
+    if (ctx->prop != ACE_PROP_INVARIANT) return set_err("listed product-DAG evaluation: Invariant bases only");
+    if (ctx->rec_dispatch && ctx->nrec == 0)
+        return set_err("ace_set_kernel mode 2: per-record dispatch needs correlation order <= 4");
     if (ctx->rec_et != 64) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     WaveCfg* wc = ctx->wcfg[ctx->cur_w];
     const int64_t ntile = (nenv + 63) / 64;
@@ -1522,6 +1550,7 @@ int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estrid
     ra.segw = ctx->rec_dispatch ? nullptr : ctx->segw.p;
     ra.segt = reinterpret_cast<const uint4*>(ctx->segt.p);
     ra.seg_ptr = ctx->seg_ptr.p;
+    ra.deep = ctx->maxorder > 4;
     ra.tasks = ctx->d_tasks.p;
     ra.task_const = ctx->task_const.p;
     ra.wtask_ptr = forces ? wc->d_ptr_all.p : wc->d_ptr_e.p;

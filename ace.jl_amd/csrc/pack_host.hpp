@@ -282,6 +282,7 @@ struct PackCopy {
     std::vector<int32_t> rec_leaf;
     std::vector<int32_t> leaf_fid;  // [nleaves] factor id (k | 0x8000 conjugated)
     std::vector<int32_t> tasks;  // [ntasks][5] (run_b, run_e, slot, rec_b, rec_e)
+    std::vector<int32_t> runs;   // [nruns][ACE_RUN_W] depth, jstart, nleaf, leaf_off, prefix factors
     std::vector<int32_t> rowptr, cols, leaf_term_ptr, task_term_ptr, term_row, term_conj, orders;
     std::vector<double> vals, term_scale;
     // class-split EuclideanVector programs: row classes and class vectors
@@ -315,6 +316,7 @@ inline PackCopy copy_pack(const ace_pack* pk) {
     p.rec_hdr.assign(pk->rec_hdr, pk->rec_hdr + 8 * (size_t)pk->nrec);
     p.rec_leaf.assign(pk->rec_leaf, pk->rec_leaf + 4 * (size_t)pk->nrec);
     p.tasks.assign(pk->tasks, pk->tasks + 5 * (size_t)pk->ntasks);
+    p.runs.assign(pk->runs, pk->runs + (size_t)ACE_RUN_W * pk->nruns);
     p.classed = pk->classed;
     if (p.classed) {
         p.row_class = pack_row_classes(pk);
@@ -460,15 +462,31 @@ inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint
     return "";
 }
 
-// The same records as kind-sorted segments of self-contained records (the
-// segment kernel): per task, its records grouped by kind = depth * 4 + leaf
-// slots - 1 (stable within a kind), one segment per kind present.  Record
-// words: flags (bit 31 conjugated leaves, bit 31 - t conjugated prefix factor
-// t), the depth prefix offsets, the leaf slot offsets, padding to an even
-// count, then the slot coefficients (FP64, or complex for class-split
-// programs).  half: 32-environment tiles, leaf slots rounded up to 2 or 4
-// (the two half-waves take alternate slots); padding slots repeat leaf 0's
-// offset with coefficient 0.
+// Leaves per record of the record kernels: runs longer than this continue in
+// further records with the same prefix.
+constexpr int SEG_LEAVES = 4;
+// Deepest prefix of a segment record (correlation order 8).
+constexpr int SEG_MAXD = 7;
+
+// Records of task t when its runs are cut into records of at most SEG_LEAVES
+// leaves (the task's cost in the task lists).
+inline int task_records(const PackCopy& p, int t) {
+    int n = 0;
+    for (int r = p.tasks[5 * (size_t)t]; r < p.tasks[5 * (size_t)t + 1]; ++r)
+        n += (std::max(p.runs[(size_t)ACE_RUN_W * r + 2], 1) + SEG_LEAVES - 1) / SEG_LEAVES;
+    return n;
+}
+
+// The programs as kind-sorted segments of self-contained records (the
+// segment kernel).  Every run is cut into records of at most SEG_LEAVES
+// leaves; a record carries its whole prefix, so the records of a task can be
+// grouped by kind = depth * 4 + leaf slots - 1 (stable within a kind), one
+// segment per kind present.  Record words: flags (bit 31 conjugated leaves,
+// bit 31 - t conjugated prefix factor t), the depth prefix offsets, the leaf
+// slot offsets, padding to an even count, then the slot coefficients (FP64,
+// or complex for class-split programs).  half: 32-environment tiles, leaf
+// slots rounded up to 2 or 4 (the two half-waves take alternate slots);
+// padding slots repeat leaf 0's offset with coefficient 0.
 //   segt[4 * s]     kind, record count, word offset of the first record, 0
 //   seg_ptr[t]      the segments of task t: [seg_ptr[t], seg_ptr[t + 1])
 struct Segments {
@@ -482,45 +500,50 @@ inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, 
     out.seg_ptr.assign(1, 0);
     auto off = [&](uint32_t f) { return (f & 0x7fffu) << lds_shift; };
     const int CW = p.classed ? 4 : 2;  // coefficient words per slot
+    constexpr int NK = 4 * (SEG_MAXD + 1);
+    struct Rec {
+        int run, q0, n;
+    };
     for (int t = 0; t < p.ntasks; ++t) {
-        const int rb = p.tasks[5 * (size_t)t + 3], re = p.tasks[5 * (size_t)t + 4];
-        std::vector<std::vector<int>> by_kind(16);
-        for (int r = rb; r < re; ++r) {
-            const uint32_t* h = p.rec_hdr.data() + 8 * (size_t)r;
-            const int d = h[0] & 3, nl = (h[0] >> 5) & 7;
-            if (nl < 1 || nl > 4) return "record without leaves";
-            const int ns = half ? (nl <= 2 ? 2 : 4) : nl;
-            by_kind[d * 4 + ns - 1].push_back(r);
+        std::vector<std::vector<Rec>> by_kind(NK);
+        for (int r = p.tasks[5 * (size_t)t]; r < p.tasks[5 * (size_t)t + 1]; ++r) {
+            const int32_t* run = p.runs.data() + (size_t)ACE_RUN_W * r;
+            const int d = run[0], nl = run[2];
+            if (d < 0 || d > SEG_MAXD) return "run prefix deeper than 7 factors";
+            if (nl < 1) return "run without leaves";
+            for (int q0 = 0; q0 < nl; q0 += SEG_LEAVES) {
+                const int n = std::min(SEG_LEAVES, nl - q0);
+                const int ns = half ? (n <= 2 ? 2 : 4) : n;
+                by_kind[d * 4 + ns - 1].push_back({r, q0, n});
+            }
         }
-        for (int kind = 0; kind < 16; ++kind) {
-            const std::vector<int>& rs = by_kind[kind];
+        for (int kind = 0; kind < NK; ++kind) {
+            const std::vector<Rec>& rs = by_kind[kind];
             if (rs.empty()) continue;
             const int d = kind / 4, ns = kind % 4 + 1;
             const int hw = (2 + d + ns) & ~1, rw = hw + ns * CW;
             out.segt.insert(out.segt.end(), {(uint32_t)kind, (uint32_t)rs.size(), (uint32_t)out.words.size(), 0u});
-            for (int r : rs) {
-                const uint32_t* h = p.rec_hdr.data() + 8 * (size_t)r;
-                const int nl = (h[0] >> 5) & 7;
-                const uint32_t lid[4] = {h[3] & 0xffffu, h[3] >> 16, h[4] & 0xffffu, h[4] >> 16};
-                const uint32_t fid[3] = {h[1] & 0xffffu, h[1] >> 16, h[2] & 0xffffu};
-                const bool cj = (lid[0] & 0x8000u) != 0;
-                for (int i = 0; i < nl; ++i)
-                    if (((lid[i] & 0x8000u) != 0) != cj) return "record leaves mix conjugation";
+            for (const Rec& rc : rs) {
+                const int32_t* run = p.runs.data() + (size_t)ACE_RUN_W * rc.run;
+                const int lo = run[3] + rc.q0;
+                const bool cj = (p.leaf_fid[lo] & 0x8000) != 0;
+                for (int i = 0; i < rc.n; ++i)
+                    if (((p.leaf_fid[lo + i] & 0x8000) != 0) != cj) return "record leaves mix conjugation";
                 std::vector<uint32_t> w((size_t)rw, 0u);
                 uint32_t fl = cj ? 0x80000000u : 0u;
                 for (int q = 0; q < d; ++q) {
-                    if (fid[q] & 0x8000u) fl |= 0x80000000u >> (q + 1);
-                    w[1 + q] = off(fid[q]);
+                    const uint32_t f = (uint32_t)run[4 + q];
+                    if (f & 0x8000u) fl |= 0x80000000u >> (q + 1);
+                    w[1 + q] = off(f);
                 }
                 w[0] = fl;
-                for (int i = 0; i < ns; ++i) w[1 + d + i] = off(lid[i < nl ? i : 0]);
-                for (int i = 0; i < nl; ++i) {
-                    const int lf = p.rec_leaf[4 * (size_t)r + i];
-                    if (lf < 0) return "record leaf slot without a leaf";
+                for (int i = 0; i < ns; ++i) w[1 + d + i] = off((uint32_t)p.leaf_fid[lo + (i < rc.n ? i : 0)]);
+                for (int i = 0; i < rc.n; ++i) {
+                    const size_t lf = (size_t)lo + i;
                     double c[2];
                     if (p.classed) {
-                        c[0] = cf.leaf[2 * (size_t)lf];
-                        c[1] = cj ? -cf.leaf[2 * (size_t)lf + 1] : cf.leaf[2 * (size_t)lf + 1];
+                        c[0] = cf.leaf[2 * lf];
+                        c[1] = cj ? -cf.leaf[2 * lf + 1] : cf.leaf[2 * lf + 1];
                     } else {
                         c[0] = cf.leaf[lf];
                     }
@@ -549,7 +572,7 @@ inline std::vector<int> mode_tasks(const PackCopy& p, bool forces) {
 // to the least loaded wave), every list ascending: ptr[V + 1], list.
 inline void lpt_lists(const PackCopy& p, const std::vector<int>& tasks, int V, std::vector<int32_t>& ptr,
                       std::vector<int32_t>& list) {
-    auto cost = [&](int t) { return std::max(1, p.tasks[5 * (size_t)t + 4] - p.tasks[5 * (size_t)t + 3]); };
+    auto cost = [&](int t) { return std::max(1, task_records(p, t)); };
     std::vector<int> order(tasks);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
     std::vector<long> load(V, 0);

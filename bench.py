@@ -70,7 +70,8 @@ def executed_flops(pack, forces=True, kernel=0):
     """FLOPs the product-DAG kernel executes per environment.
 
     Segment kernel (kernel code 0 / 1: 64- / 32-environment tiles), per record
-    of depth d with s leaf slots: 4 per slot, less 2 for the first (a
+    (a run's leaves cut into records of at most 4) of depth d with s leaf
+    slots: 4 per slot, less 2 for the first (a
     multiply), 8 for the complex coefficients of class-split EuclideanVector
     programs (less 4); the whole prefix product, 6 per factor after the first;
     8 for the accumulation (2 without a prefix).  On 32-environment tiles the
@@ -84,9 +85,14 @@ def executed_flops(pack, forces=True, kernel=0):
         per_leaf = 8.0 if cplx else 4.0
         return per_leaf * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
     tasks = list(pack.energy_tasks) + (list(pack.grad_tasks) if forces else [])
-    recs = np.concatenate([np.arange(pack.tasks[t][3], pack.tasks[t][4]) for t in tasks]) if tasks else []
-    h = pack.rec_hdr[recs, 0].astype(np.int64)
-    d, nl = h & 3, (h >> 5) & 7
+    runs = np.concatenate([np.arange(pack.tasks[t][0], pack.tasks[t][1]) for t in tasks]) if tasks else []
+    rd, rn = pack.runs[runs, 0].astype(np.int64), pack.runs[runs, 2].astype(np.int64)
+    # records: every run cut into records of at most 4 leaves
+    nrec = (rn + 3) // 4
+    d = np.repeat(rd, nrec)
+    first = np.repeat(np.cumsum(nrec) - nrec, nrec)
+    q = np.arange(len(d)) - first                         # record index within its run
+    nl = np.minimum(4, np.repeat(rn, nrec) - 4 * q)
     lanes = 2 if kernel == 1 else 1
     slots = np.where(nl <= 2, 2, 4) if kernel == 1 else nl
     per_slot = 8.0 if cplx else 4.0

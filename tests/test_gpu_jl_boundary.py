@@ -12,7 +12,7 @@ import pytest
 import torch
 
 import ace_oracle as O
-from acehip import synth
+from acehip import _lib, synth
 from acehip.evaluator import EnvBatch, HIPEvaluator
 from helpers import basis_for, random_batch, rel_err
 
@@ -65,7 +65,10 @@ def test_jl_tables_bit_identical_to_python_pack(cuda, name, args, kw, wid, nenv)
 
 def test_order5_basis_vs_oracle(cuda):
     """Correlation order 5 (degree 6, L 3): prefixes of four factors, run by
-    the interpreter with seven prefix levels; 200 environments = 4 tiles."""
+    the segment kernel (records with deep prefixes) and by the interpreter
+    with seven prefix levels; 200 environments = 4 tiles, with the tiles'
+    tasks split over 2 workgroups too.  The per-record dispatch kernel has no
+    records for such bases and says so."""
     print("[gpu] order-5 basis vs oracle", flush=True)
     basis, T = basis_for(5, 6, 3)
     assert max(T["orders"]) == 5
@@ -73,11 +76,22 @@ def test_order5_basis_vs_oracle(cuda):
     off, R, _ = random_batch(200, 41, nmin=4, nmax=30)
     ev = HIPEvaluator(basis, c, device=0, waves=8)
     b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
-    E, dE = ev.energy_forces(b)
-    torch.cuda.synchronize()
-    assert ev.last_dag_shape()[2] == 64
-    assert rel_err(E.cpu().numpy(), O.energy(T, c, off, R)) < E_TOL
-    assert rel_err(dE.cpu().numpy().T, O.grad_config(T, c, off, R)) < F_TOL
+    E_ref, g_ref = O.energy(T, c, off, R), O.grad_config(T, c, off, R)
+    for mode, parts, kern in ((0, 0, 0), (0, 2, 0), (1, 0, 64)):
+        ev.set_kernel(mode)
+        ev.set_parts(parts)
+        E, dE = ev.energy_forces(b)
+        E1 = ev.energy(b)
+        torch.cuda.synchronize()
+        assert ev.last_dag_shape()[2] == kern
+        assert rel_err(E.cpu().numpy(), E_ref) < E_TOL
+        assert rel_err(E1.cpu().numpy(), E_ref) < E_TOL
+        assert rel_err(dE.cpu().numpy().T, g_ref) < F_TOL
+    ev.set_kernel(2)
+    ev.set_parts(0)
+    with pytest.raises(_lib.AceError, match="correlation order"):
+        ev.energy_forces(b)
+    ev.set_kernel(0)
     Bv = ev.basis_B(b).cpu().numpy()
     B_ref = O.eval_B(T, O.eval_AA(T, O.eval_A(T, off, R)))
     assert rel_err(Bv, B_ref) < E_TOL
