@@ -510,8 +510,8 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __rest
 // k_dags: the record programs regrouped into kind-sorted segments of
 // self-contained records.  Every record recomputes its whole prefix product,
 // so the records of a task can be reordered: each task is a short list of
-// segments, each a run of records of one kind (prefix depth D = 0..3, NL =
-// 1..4 leaf slots), and the kernel switches once per segment, not once per
+// segments, each a run of records of one kind (prefix depth D = 0..7, NL =
+// 1..8 leaf slots), and the kernel switches once per segment, not once per
 // record.  The loop over a segment has no branches, so the LDS reads of
 // consecutive records overlap.  Record words (flags, prefix offsets, leaf
 // offsets, padding to an even count, coefficients):
@@ -519,8 +519,8 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __rest
 //                     factor t (1..3) is conjugated
 //   w[1 .. D]         LDS byte offsets of the prefix factors
 //   w[1 + D ..]       LDS byte offsets of the NL leaves (32-environment
-//                     tiles: NL = 2 or 4, the low half-wave takes the even
-//                     leaf slots and the high half the odd ones)
+//                     tiles: NL even, the low half-wave takes the even leaf
+//                     slots and the high half the odd ones)
 //   then NL FP64 coefficients (CPLX: NL complex), zero for padding slots.
 template <int D, int NL, bool CPLX>
 struct SegShape {
@@ -543,8 +543,9 @@ __device__ __forceinline__ void dags_segment(const char* lds, const uint32_t* wp
     constexpr int NV = HALF ? NL / 2 : NL;  // leaves per lane
     auto at = [&](uint32_t off) { return *reinterpret_cast<const double2*>(lds + off); };
     const SegWord* w = (const SegWord*)(uintptr_t)wp;
-    // unrolled by 3 (real coefficients) or 2 (complex): measured fastest
-#pragma unroll(CPLX ? 2 : 3)
+    // unrolled by 3 (real coefficients) or 2 (complex), by one less for
+    // records of more than 4 leaf slots (register use): measured fastest
+#pragma unroll(CPLX ? (NL > 4 ? 1 : 2) : (NL > 4 ? 2 : 3))
     for (int i = 0; i < count; ++i, w += RW) {
         uint32_t h[RW];
 #pragma unroll
@@ -656,34 +657,29 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
             const uint32_t kind = st[0], count = st[1], off = st[2];
             const uint32_t* wp = segw + off;
 #define ACE_SEG(DD, NN)                                                                  \
-    case DD * 4 + NN - 1:                                                                \
+    case DD * 8 + NN - 1:                                                                \
         dags_segment<DD, NN, CPLX, HALF>(lds, wp, (int)count, hi, acc);                  \
         break;
-            // DEEP: prefixes of 4 to 7 factors (correlation orders 5 to 8),
-            // 64-environment tiles only
+#define ACE_SEG4(DD) ACE_SEG(DD, 1) ACE_SEG(DD, 2) ACE_SEG(DD, 3) ACE_SEG(DD, 4)
+#define ACE_SEG8(DD) ACE_SEG4(DD) ACE_SEG(DD, 5) ACE_SEG(DD, 6) ACE_SEG(DD, 7) ACE_SEG(DD, 8)
+#define ACE_SEGH(DD) ACE_SEG(DD, 2) ACE_SEG(DD, 4) ACE_SEG(DD, 6) ACE_SEG(DD, 8)
+            // kind = depth * 8 + leaf slots - 1.  Records of up to 8 leaves
+            // (32-environment tiles: 2, 4, 6 or 8 slots); DEEP: prefixes of 4
+            // to 7 factors (correlation orders 5 to 8), records of up to 4
+            // leaves, 64-environment tiles only
             static_assert(!(HALF && DEEP), "deep prefixes run on 64-environment tiles");
             if constexpr (HALF) {
-                switch (kind) { ACE_SEG(0, 2) ACE_SEG(0, 4) ACE_SEG(1, 2) ACE_SEG(1, 4)
-                                ACE_SEG(2, 2) ACE_SEG(2, 4) ACE_SEG(3, 2) ACE_SEG(3, 4) }
+                switch (kind) { ACE_SEGH(0) ACE_SEGH(1) ACE_SEGH(2) ACE_SEGH(3) }
             } else if constexpr (DEEP) {
                 switch (kind) {
-                    ACE_SEG(0, 1) ACE_SEG(0, 2) ACE_SEG(0, 3) ACE_SEG(0, 4)
-                    ACE_SEG(1, 1) ACE_SEG(1, 2) ACE_SEG(1, 3) ACE_SEG(1, 4)
-                    ACE_SEG(2, 1) ACE_SEG(2, 2) ACE_SEG(2, 3) ACE_SEG(2, 4)
-                    ACE_SEG(3, 1) ACE_SEG(3, 2) ACE_SEG(3, 3) ACE_SEG(3, 4)
-                    ACE_SEG(4, 1) ACE_SEG(4, 2) ACE_SEG(4, 3) ACE_SEG(4, 4)
-                    ACE_SEG(5, 1) ACE_SEG(5, 2) ACE_SEG(5, 3) ACE_SEG(5, 4)
-                    ACE_SEG(6, 1) ACE_SEG(6, 2) ACE_SEG(6, 3) ACE_SEG(6, 4)
-                    ACE_SEG(7, 1) ACE_SEG(7, 2) ACE_SEG(7, 3) ACE_SEG(7, 4)
+                    ACE_SEG4(0) ACE_SEG4(1) ACE_SEG4(2) ACE_SEG4(3) ACE_SEG4(4) ACE_SEG4(5) ACE_SEG4(6) ACE_SEG4(7)
                 }
             } else {
-                switch (kind) {
-                    ACE_SEG(0, 1) ACE_SEG(0, 2) ACE_SEG(0, 3) ACE_SEG(0, 4)
-                    ACE_SEG(1, 1) ACE_SEG(1, 2) ACE_SEG(1, 3) ACE_SEG(1, 4)
-                    ACE_SEG(2, 1) ACE_SEG(2, 2) ACE_SEG(2, 3) ACE_SEG(2, 4)
-                    ACE_SEG(3, 1) ACE_SEG(3, 2) ACE_SEG(3, 3) ACE_SEG(3, 4)
-                }
+                switch (kind) { ACE_SEG8(0) ACE_SEG8(1) ACE_SEG8(2) ACE_SEG8(3) }
             }
+#undef ACE_SEGH
+#undef ACE_SEG8
+#undef ACE_SEG4
 #undef ACE_SEG
         }
         if (HALF) {
@@ -818,7 +814,8 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
     }
     if (ctx->rec_et > 0) {
         acehost::Segments sg;
-        const std::string serr = acehost::segments(ctx->host, cf, ctx->rec_et == 64 ? 10 : 9, ctx->rec_et == 32, sg);
+        const std::string serr = acehost::segments(ctx->host, cf, ctx->rec_et == 64 ? 10 : 9, ctx->rec_et == 32,
+                                                   acehost::seg_leaves(ctx->maxorder), sg);
         if (!serr.empty()) return set_err(serr);
         if (ctx->segw.upload(sg.words.data(), sg.words.size()) || ctx->segt.upload(sg.segt.data(), sg.segt.size()) ||
             ctx->seg_ptr.upload(sg.seg_ptr.data(), sg.seg_ptr.size()))

@@ -462,11 +462,15 @@ inline std::string records(const PackCopy& p, const Coeffs& cf, std::vector<uint
     return "";
 }
 
-// Leaves per record of the record kernels: runs longer than this continue in
-// further records with the same prefix.
+// Leaves per record of the per-record kernel (and the record count that
+// stands for a task's cost in the task lists): runs longer than this continue
+// in further records with the same prefix.
 constexpr int SEG_LEAVES = 4;
 // Deepest prefix of a segment record (correlation order 8).
 constexpr int SEG_MAXD = 7;
+// Leaves per segment record: 8 for prefixes of at most 3 factors, 4 for
+// deeper ones.
+inline int seg_leaves(int maxorder) { return maxorder > 4 ? 4 : 8; }
 
 // Records of task t when its runs are cut into records of at most SEG_LEAVES
 // leaves (the task's cost in the task lists).
@@ -478,15 +482,15 @@ inline int task_records(const PackCopy& p, int t) {
 }
 
 // The programs as kind-sorted segments of self-contained records (the
-// segment kernel).  Every run is cut into records of at most SEG_LEAVES
+// segment kernel).  Every run is cut into records of at most maxleaves
 // leaves; a record carries its whole prefix, so the records of a task can be
-// grouped by kind = depth * 4 + leaf slots - 1 (stable within a kind), one
+// grouped by kind = depth * 8 + leaf slots - 1 (stable within a kind), one
 // segment per kind present.  Record words: flags (bit 31 conjugated leaves,
 // bit 31 - t conjugated prefix factor t), the depth prefix offsets, the leaf
 // slot offsets, padding to an even count, then the slot coefficients (FP64,
 // or complex for class-split programs).  half: 32-environment tiles, leaf
-// slots rounded up to 2 or 4 (the two half-waves take alternate slots);
-// padding slots repeat leaf 0's offset with coefficient 0.
+// slots rounded up to an even count (the two half-waves take alternate
+// slots); padding slots repeat leaf 0's offset with coefficient 0.
 //   segt[4 * s]     kind, record count, word offset of the first record, 0
 //   seg_ptr[t]      the segments of task t: [seg_ptr[t], seg_ptr[t + 1])
 struct Segments {
@@ -494,13 +498,15 @@ struct Segments {
     std::vector<int32_t> seg_ptr;
 };
 
-inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, bool half, Segments& out) {
+inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, bool half, int maxleaves,
+                            Segments& out) {
+    if (maxleaves < 1 || maxleaves > 8) return "segment records hold 1 to 8 leaves";
     out.words.clear();
     out.segt.clear();
     out.seg_ptr.assign(1, 0);
     auto off = [&](uint32_t f) { return (f & 0x7fffu) << lds_shift; };
     const int CW = p.classed ? 4 : 2;  // coefficient words per slot
-    constexpr int NK = 4 * (SEG_MAXD + 1);
+    constexpr int NK = 8 * (SEG_MAXD + 1);
     struct Rec {
         int run, q0, n;
     };
@@ -511,16 +517,16 @@ inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, 
             const int d = run[0], nl = run[2];
             if (d < 0 || d > SEG_MAXD) return "run prefix deeper than 7 factors";
             if (nl < 1) return "run without leaves";
-            for (int q0 = 0; q0 < nl; q0 += SEG_LEAVES) {
-                const int n = std::min(SEG_LEAVES, nl - q0);
-                const int ns = half ? (n <= 2 ? 2 : 4) : n;
-                by_kind[d * 4 + ns - 1].push_back({r, q0, n});
+            for (int q0 = 0; q0 < nl; q0 += maxleaves) {
+                const int n = std::min(maxleaves, nl - q0);
+                const int ns = half ? (n + 1) & ~1 : n;
+                by_kind[d * 8 + ns - 1].push_back({r, q0, n});
             }
         }
         for (int kind = 0; kind < NK; ++kind) {
             const std::vector<Rec>& rs = by_kind[kind];
             if (rs.empty()) continue;
-            const int d = kind / 4, ns = kind % 4 + 1;
+            const int d = kind / 8, ns = kind % 8 + 1;
             const int hw = (2 + d + ns) & ~1, rw = hw + ns * CW;
             out.segt.insert(out.segt.end(), {(uint32_t)kind, (uint32_t)rs.size(), (uint32_t)out.words.size(), 0u});
             for (const Rec& rc : rs) {
