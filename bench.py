@@ -24,12 +24,12 @@ METRIC = "env/s energy+forces: plain FP64 vs surrogate at ≤1 meV/atom MAE"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per product-DAG launch on this workload (100k environments, 16-wave
-# tiles split over 3 workgroups, 4 for the tiles of the last round), from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the
-# factors a 256 MiB device copy gives the same counters (2.0 for FETCH_SIZE,
-# 1.0 for WRITE_SIZE): 720.8 MB read + 269.7 MB written.  Only valid for the
-# default --nenv; re-measured whenever the kernel changes.
-DAG_TRAFFIC_BYTES = {100_000: 990.4e6}
+# tiles split over 3 workgroups), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script, scaled by the factors a 256 MiB device copy
+# gives the same counters (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE): 715.6 MB
+# read + 269.7 MB written.  Only valid for the default --nenv; re-measured
+# whenever the kernel changes.
+DAG_TRAFFIC_BYTES = {100_000: 985.3e6}
 WORKLOAD_NAMES = {
     1: "plain ACE energy+forces, 1000 envs x 12 neighbours, order 3, degree 6, Lmax 4, Invariant, FP64",
     2: "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], order 4, degree 12, Lmax 6, "
@@ -70,12 +70,12 @@ def executed_flops(pack, forces=True, kernel=0):
     """FLOPs the product-DAG kernel executes per environment.
 
     Segment kernel (kernel code 0 / 1: 64- / 32-environment tiles), per record
-    (a run's leaves cut into records of at most 4) of depth d with s leaf
-    slots: 4 per slot, less 2 for the first (a
+    (a run's leaves cut into records of at most 8, 4 above correlation order
+    4) of depth d with s leaf slots: 4 per slot, less 2 for the first (a
     multiply), 8 for the complex coefficients of class-split EuclideanVector
     programs (less 4); the whole prefix product, 6 per factor after the first;
     8 for the accumulation (2 without a prefix).  On 32-environment tiles the
-    slots are rounded up to 2 or 4 and both half-waves of an environment
+    slots are rounded up to an even count and both half-waves of an environment
     compute the prefix and the accumulation.  Run interpreter: 4 per leaf, 6
     per new prefix product, 8 per run."""
     import numpy as np
@@ -87,14 +87,16 @@ def executed_flops(pack, forces=True, kernel=0):
     tasks = list(pack.energy_tasks) + (list(pack.grad_tasks) if forces else [])
     runs = np.concatenate([np.arange(pack.tasks[t][0], pack.tasks[t][1]) for t in tasks]) if tasks else []
     rd, rn = pack.runs[runs, 0].astype(np.int64), pack.runs[runs, 2].astype(np.int64)
-    # records: every run cut into records of at most 4 leaves
-    nrec = (rn + 3) // 4
+    # records: every run cut into records of at most 8 leaves (4 for
+    # correlation orders above 4)
+    ch = 4 if pack.maxorder > 4 else 8
+    nrec = (rn + ch - 1) // ch
     d = np.repeat(rd, nrec)
     first = np.repeat(np.cumsum(nrec) - nrec, nrec)
     q = np.arange(len(d)) - first                         # record index within its run
-    nl = np.minimum(4, np.repeat(rn, nrec) - 4 * q)
+    nl = np.minimum(ch, np.repeat(rn, nrec) - ch * q)
     lanes = 2 if kernel == 1 else 1
-    slots = np.where(nl <= 2, 2, 4) if kernel == 1 else nl
+    slots = (nl + 1) // 2 * 2 if kernel == 1 else nl
     per_slot = 8.0 if cplx else 4.0
     leaf = per_slot * slots - per_slot / 2 * lanes
     pre = 6.0 * np.maximum(d - 1, 0) + np.where(d > 0, 8.0, 2.0)
