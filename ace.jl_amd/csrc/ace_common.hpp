@@ -33,19 +33,27 @@
 __host__ __device__ constexpr int idxp(int l, int m) { return m + l * (l + 1) / 2; }
 
 // Per-neighbour parameters of the 1p basis: the radial recursion
-// coefficients and envelope (orthpolys.jl:82-99, distancetransforms.jl:73),
-// and the ALP recursion coefficients a_lm = alpA[idxp(l, m)],
-// b_lm = alpB[idxp(l, m)] of sphericalharmonics.jl:146-159 (a device table
-// owned by the context, so the library keeps no per-device global state).
+// coefficients and envelope (orthpolys.jl:82-99, distancetransforms.jl:73).
 struct RadialParams {
     const double* A;
     const double* B;
     const double* C;
-    const double* alpA;
-    const double* alpB;
     double r0, tl, tr;
     int nr, tp, pl, pr;
 };
+
+// ALP recursion coefficients a_lm and b_lm (sphericalharmonics.jl:146-159).
+// Every caller's loops over l and m are unrolled, so these fold to
+// compile-time constants (IEEE square roots, the same bits as the reference's
+// table): no table, no loads.
+__device__ __forceinline__ double alp_a(int l, int m) {
+    const double ls = (double)l * l, ms = (double)m * m;
+    return sqrt((4 * ls - 1.0) / (ls - ms));
+}
+__device__ __forceinline__ double alp_b(int l, int m) {
+    const double ls = (double)(l - 1) * (l - 1), ms = (double)m * m;
+    return -sqrt((ls - ms) / (4 * ls - 1.0));
+}
 
 __device__ __forceinline__ double ipow(double x, int p) {
     double y = 1.0;
@@ -115,14 +123,16 @@ __device__ __forceinline__ void alp_eval_ed(const RadialParams& rp, const Sph& S
     for (int l = 2; l <= L; ++l) {
         {
             const int i = idxp(l, 0), i1 = idxp(l - 1, 0), i2 = idxp(l - 2, 0);
-            P[i] = rp.alpA[i] * (S.cth * P[i1] + rp.alpB[i] * P[i2]);
-            dP[i] = rp.alpA[i] * (-S.sth * P[i1] + S.cth * dP[i1] + rp.alpB[i] * dP[i2]);
+            const double a = alp_a(l, 0), b = alp_b(l, 0);
+            P[i] = a * (S.cth * P[i1] + b * P[i2]);
+            dP[i] = a * (-S.sth * P[i1] + S.cth * dP[i1] + b * dP[i2]);
         }
 #pragma unroll
         for (int m = 1; m <= l - 2; ++m) {
             const int i = idxp(l, m), i1 = idxp(l - 1, m), i2 = idxp(l - 2, m);
-            P[i] = rp.alpA[i] * (S.cth * P[i1] + rp.alpB[i] * P[i2]);
-            dP[i] = rp.alpA[i] * (-st2 * P[i1] + S.cth * dP[i1] + rp.alpB[i] * dP[i2]);
+            const double a = alp_a(l, m), b = alp_b(l, m);
+            P[i] = a * (S.cth * P[i1] + b * P[i2]);
+            dP[i] = a * (-st2 * P[i1] + S.cth * dP[i1] + b * dP[i2]);
         }
         const double c = sqrt(2.0 * (l - 1) + 3.0);
         P[idxp(l, l - 1)] = c * S.cth * temp1;
@@ -216,15 +226,14 @@ struct WaveCfg {
 
 constexpr int FORCE_NRMAX = 16;  // radial functions held in registers by the force kernels
 constexpr int FF_QMAX = 256;  // (l, m >= 0, n) entries of the flat force kernel (npool <= 256)
-// environments a 256-neighbour block stages in LDS: 32 when their G fits in
-// FF_LDS (short environments), else 16
+// environments a 256-neighbour block stages in LDS per round: 32 when their G
+// fits in FF_LDS (short environments), else 16
 constexpr int FF_MAXE = 16;
 constexpr int FF_MAXE_WIDE = 32;
 constexpr size_t FF_LDS = 40 * 1024;
 
 struct ForceFlatTab {
-    int8_t nrl[ACE_LMAX + 1];                                  // n bound per l
-    int16_t slot[(ACE_LMAX + 1) * (ACE_LMAX + 2) / 2][FORCE_NRMAX];  // G slot of (l, m >= 0, n)
+    int8_t nrl[ACE_LMAX + 1];  // n bound per l
     int16_t nq;  // number of (l, m, n) entries (their G slots in staging order: ace_ctx::ffq)
 };
 
@@ -262,7 +271,6 @@ struct ace_ctx {
     acehost::PackCopy host;
     // device
     DevBuf<double> rA, rB, rC;
-    DevBuf<double> alp;  // ALP recursion coefficients a_lm, then b_lm ([2][ACE_NPMAX])
     DevBuf<int32_t> pool, aspec_fid, aspec4, orders, iAA2iA, rowptr, cols;
     DevBuf<double> aspec_sgn, vals;
     DevBuf<BtRec> bt_rec;
@@ -366,8 +374,6 @@ inline RadialParams radial_params(const ace_ctx* ctx) {
     rp.A = ctx->rA.p;
     rp.B = ctx->rB.p;
     rp.C = ctx->rC.p;
-    rp.alpA = ctx->alp.p;
-    rp.alpB = ctx->alp.p + ACE_NPMAX;
     rp.r0 = ctx->r0;
     rp.tl = ctx->tl;
     rp.tr = ctx->tr;

@@ -49,6 +49,42 @@ __device__ __forceinline__ void gather_g(const int32_t* __restrict__ gslot_ptr,
     }
 }
 
+// The force contraction Re sum_{l,m,n} G[l,m,n] grad(R_n Y_lm)(r_j),
+// factored over the radial index: with a = sum_n G dR_n/dt and b = sum_n G
+// R_n (complex; 4 fused multiply-adds per n) the (l, m) term is
+// Re(a Y) rhat dt/dr + Re(b grad Y), whose angular part needs only the
+// azimuthal and polar derivative factors of Y (cYlm_ed! and dspher_to_dcart,
+// sphericalharmonics.jl:60-65,410-443; Pl = P_lm / sin(theta) for m >= 1).
+// force_lm_add adds one (l, m) term to the gradient.
+__device__ __forceinline__ void force_lm_add(double Pl, double dPl, int m, double epr, double epi, const Sph& S,
+                                             double dtdr, double rinv, double rx, double ry, double rz, double ax,
+                                             double ay, double bx, double by, double& g0, double& g1, double& g2) {
+    double sr, su, sv;  // Re(a Y), Re(b fp), Re(b ft): fp = dY/dphi / sin(theta), ft = dY/dtheta
+    if (m == 0) {
+        constexpr double isq2 = 0.7071067811865475;
+        sr = isq2 * Pl * ax;
+        su = 0.0;
+        sv = isq2 * dPl * bx;
+    } else {
+        sr = S.sth * Pl * (epr * ax - epi * ay);
+        su = -(double)m * Pl * (epi * bx + epr * by);
+        sv = dPl * (epr * bx - epi * by);
+    }
+    const double a = dtdr * sr, ct = S.cth * sv;
+    g0 += fma(a, rx, rinv * (S.cphi * ct - S.sphi * su));
+    g1 += fma(a, ry, rinv * (S.sphi * ct + S.cphi * su));
+    g2 += fma(a, rz, rinv * (-S.sth * sv));
+}
+
+// The running gradient sums materialised after every (l, m) term: without
+// this the compiler defers some of the sums to the end of the kernel and
+// spills every term's inputs (measured: hundreds of spilled registers).
+template <int NC>
+__device__ __forceinline__ void force_keep(double (&g)[NC][3]) {
+#pragma unroll
+    for (int p = 0; p < NC; ++p) asm volatile("" : "+v"(g[p][0]), "+v"(g[p][1]), "+v"(g[p][2]));
+}
+
 template <int L, int PROP>
 __global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
                                                const int32_t* __restrict__ off,
@@ -127,45 +163,28 @@ __global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
 #pragma unroll
             for (int l = m; l <= L; ++l) {
                 const int ip = idxp(l, m);
-                // Y and its Cartesian gradient (cYlm_ed!, dspher_to_dcart)
-                double Yr, Yi, fpr, fpi, ftr, fti;
-                if (m == 0) {
-                    Yr = P[ip] * 0.7071067811865475;
-                    Yi = 0.0;
-                    fpr = fpi = 0.0;
-                    ftr = dP[ip] * 0.7071067811865475;
-                    fti = 0.0;
-                } else {
-                    const double pds = P[ip];
-                    Yr = epr * pds * S.sth;
-                    Yi = epi * pds * S.sth;
-                    // dep/dphi = i m ep
-                    fpr = -m * epi * pds;
-                    fpi = m * epr * pds;
-                    ftr = epr * dP[ip];
-                    fti = epi * dP[ip];
-                }
-                const double dYr[3] = {(-S.sphi * fpr + S.cphi * S.cth * ftr) * rinv,
-                                       (S.cphi * fpr + S.sphi * S.cth * ftr) * rinv,
-                                       (-S.sth * ftr) * rinv};
-                const double dYi[3] = {(-S.sphi * fpi + S.cphi * S.cth * fti) * rinv,
-                                       (S.cphi * fpi + S.sphi * S.cth * fti) * rinv,
-                                       (-S.sth * fti) * rinv};
                 const int16_t* kr = kidx + ((int64_t)(mu * (L + 1) + l) * (L + 1) + m) * nr;
+                double ax[NC], ay[NC], bx[NC], by[NC];
+#pragma unroll
+                for (int p = 0; p < NC; ++p) ax[p] = ay[p] = bx[p] = by[p] = 0.0;
                 for (int n = 0; n < nr; ++n) {
                     const int k = kr[n];
                     if (k < 0) continue;
-                    const double R = myR[n], dR = myR[nr + n] * dtdr;
+                    const double R = myR[n], dR = myR[nr + n];
 #pragma unroll
                     for (int p = 0; p < NC; ++p) {
                         const double2 G = Gs[k * NC + p];
-                        // Re(G * (dR rhat Y + R dY))
-                        const double gy = G.x * Yr - G.y * Yi;
-                        g[p][0] += dR * gy * rx + R * (G.x * dYr[0] - G.y * dYi[0]);
-                        g[p][1] += dR * gy * ry + R * (G.x * dYr[1] - G.y * dYi[1]);
-                        g[p][2] += dR * gy * rz + R * (G.x * dYr[2] - G.y * dYi[2]);
+                        ax[p] = fma(G.x, dR, ax[p]);
+                        ay[p] = fma(G.y, dR, ay[p]);
+                        bx[p] = fma(G.x, R, bx[p]);
+                        by[p] = fma(G.y, R, by[p]);
                     }
                 }
+#pragma unroll
+                for (int p = 0; p < NC; ++p)
+                    force_lm_add(P[ip], dP[ip], m, epr, epi, S, dtdr, rinv, rx, ry, rz, ax[p], ay[p], bx[p], by[p],
+                                 g[p][0], g[p][1], g[p][2]);
+                force_keep<NC>(g);
             }
         }
 #pragma unroll
@@ -296,10 +315,10 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
                         dPl = c * (-st2 * Pmm + S.cth * dPmm);
                     }
                 } else {
-                    const int i = idxp(l, m);
-                    Pl = rp.alpA[i] * (S.cth * Pb + rp.alpB[i] * Pa);
-                    dPl = m == 0 ? rp.alpA[i] * (-S.sth * Pb + S.cth * dPb + rp.alpB[i] * dPa)
-                                 : rp.alpA[i] * (-st2 * Pb + S.cth * dPb + rp.alpB[i] * dPa);
+                    const double al = alp_a(l, m), bl = alp_b(l, m);
+                    Pl = al * (S.cth * Pb + bl * Pa);
+                    dPl = m == 0 ? al * (-S.sth * Pb + S.cth * dPb + bl * dPa)
+                                 : al * (-st2 * Pb + S.cth * dPb + bl * dPa);
                 }
                 if (l > m) {
                     Pa = Pb;
@@ -307,45 +326,29 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
                     Pb = Pl;
                     dPb = dPl;
                 }
-                // Y and its Cartesian gradient (cYlm_ed!, dspher_to_dcart)
-                double Yr, Yi, fpr, fpi, ftr, fti;
-                if (m == 0) {
-                    Yr = Pl * 0.7071067811865475;
-                    Yi = 0.0;
-                    fpr = fpi = 0.0;
-                    ftr = dPl * 0.7071067811865475;
-                    fti = 0.0;
-                } else {
-                    Yr = epr * Pl * S.sth;
-                    Yi = epi * Pl * S.sth;
-                    fpr = -m * epi * Pl;
-                    fpi = m * epr * Pl;
-                    ftr = epr * dPl;
-                    fti = epi * dPl;
-                }
-                const double dYr[3] = {(-S.sphi * fpr + S.cphi * S.cth * ftr) * rinv,
-                                       (S.cphi * fpr + S.sphi * S.cth * ftr) * rinv,
-                                       (-S.sth * ftr) * rinv};
-                const double dYi[3] = {(-S.sphi * fpi + S.cphi * S.cth * fti) * rinv,
-                                       (S.cphi * fpi + S.sphi * S.cth * fti) * rinv,
-                                       (-S.sth * fti) * rinv};
                 const double2* Gl = Gm + idxp(l, m) * nr * NC;
                 const int nb = ft.nrl[l];
+                double ax[NC], ay[NC], bx[NC], by[NC];
+#pragma unroll
+                for (int p = 0; p < NC; ++p) ax[p] = ay[p] = bx[p] = by[p] = 0.0;
 #pragma unroll
                 for (int n = 0; n < FORCE_NRMAX; ++n) {
                     if (n < nb) {
-                        const double Rn = R[n], dRn = dR[n] * dtdr;
 #pragma unroll
                         for (int p = 0; p < NC; ++p) {
                             const double2 G = Gl[n * NC + p];
-                            // Re(G * (dR rhat Y + R dY))
-                            const double gy = G.x * Yr - G.y * Yi;
-                            g[p][0] += dRn * gy * rx + Rn * (G.x * dYr[0] - G.y * dYi[0]);
-                            g[p][1] += dRn * gy * ry + Rn * (G.x * dYr[1] - G.y * dYi[1]);
-                            g[p][2] += dRn * gy * rz + Rn * (G.x * dYr[2] - G.y * dYi[2]);
+                            ax[p] = fma(G.x, dR[n], ax[p]);
+                            ay[p] = fma(G.y, dR[n], ay[p]);
+                            bx[p] = fma(G.x, R[n], bx[p]);
+                            by[p] = fma(G.y, R[n], by[p]);
                         }
                     }
                 }
+#pragma unroll
+                for (int p = 0; p < NC; ++p)
+                    force_lm_add(Pl, dPl, m, epr, epi, S, dtdr, rinv, rx, ry, rz, ax[p], ay[p], bx[p], by[p], g[p][0],
+                                 g[p][1], g[p][2]);
+                force_keep<NC>(g);
             }
         }
 #pragma unroll
@@ -357,16 +360,15 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
 
 // k_forcef: forces over the flat neighbour list (Invariant, one species,
 // nr <= 16).  Every lane is a neighbour, whatever environment it belongs to,
-// so no lane idles on short environments.  A lane reads its environment's
-// adjoint G[slot][env] straight from memory (lanes of one environment read the
-// same address; neighbouring environments are adjacent), with the slot of
-// every (l, m, n) in the kernel arguments.  Radial values stay in registers
-// and the ALP values are produced m-outer exactly as in k_force.
+// so no lane idles on short environments.  A block stages the adjoint G of
+// its environments in LDS, MAXE environments per round, as [(l, m, n)][env]
+// in the kernel's own (m, l, n) order; each lane computes in the round that
+// holds its environment.  Radial values stay in registers and the ALP values
+// are produced m-outer exactly as in k_force.
 
-template <int L, int NRM, bool STAGED, int MAXE>
+template <int L, int NRM, int MAXE>
 __device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceFlatTab& ft, int64_t nnbr, int64_t j,
-                                            const double* __restrict__ xyz, const double2* __restrict__ Ge,
-                                            const double2* __restrict__ Gs, int64_t estride,
+                                            const double* __restrict__ xyz, const double2* __restrict__ Gs,
                                             double* __restrict__ dE) {
     const int nr = rp.nr;
     const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
@@ -404,7 +406,7 @@ __device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceF
     const double rinv = 1.0 / (S.r + 2.220446049250313e-16);
     const double rx = x / S.r, ry = y / S.r, rz = z / S.r;
     const double st2 = S.sth * S.sth;
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    double g[1][3] = {{0.0, 0.0, 0.0}};
     int q = 0;  // running (l, m, n) index of the staged table
     double epr = 0.7071067811865475, epi = 0.0;
     double Pmm = 0.3989422804014327, dPmm = 0.0;
@@ -444,10 +446,10 @@ __device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceF
                     dPl = c * (-st2 * Pmm + S.cth * dPmm);
                 }
             } else {
-                const int i = idxp(l, m);
-                Pl = rp.alpA[i] * (S.cth * Pb + rp.alpB[i] * Pa);
-                dPl = m == 0 ? rp.alpA[i] * (-S.sth * Pb + S.cth * dPb + rp.alpB[i] * dPa)
-                             : rp.alpA[i] * (-st2 * Pb + S.cth * dPb + rp.alpB[i] * dPa);
+                const double al = alp_a(l, m), bl = alp_b(l, m);
+                Pl = al * (S.cth * Pb + bl * Pa);
+                dPl = m == 0 ? al * (-S.sth * Pb + S.cth * dPb + bl * dPa)
+                             : al * (-st2 * Pb + S.cth * dPb + bl * dPa);
             }
             if (l > m) {
                 Pa = Pb;
@@ -455,74 +457,64 @@ __device__ __forceinline__ void forcef_lane(const RadialParams& rp, const ForceF
                 Pb = Pl;
                 dPb = dPl;
             }
-            double Yr, Yi, fpr, fpi, ftr, fti;
-            if (m == 0) {
-                Yr = Pl * 0.7071067811865475;
-                Yi = 0.0;
-                fpr = fpi = 0.0;
-                ftr = dPl * 0.7071067811865475;
-                fti = 0.0;
-            } else {
-                Yr = epr * Pl * S.sth;
-                Yi = epi * Pl * S.sth;
-                fpr = -m * epi * Pl;
-                fpi = m * epr * Pl;
-                ftr = epr * dPl;
-                fti = epi * dPl;
-            }
-            const double dYr[3] = {(-S.sphi * fpr + S.cphi * S.cth * ftr) * rinv,
-                                   (S.cphi * fpr + S.sphi * S.cth * ftr) * rinv,
-                                   (-S.sth * ftr) * rinv};
-            const double dYi[3] = {(-S.sphi * fpi + S.cphi * S.cth * fti) * rinv,
-                                   (S.cphi * fpi + S.sphi * S.cth * fti) * rinv,
-                                   (-S.sth * fti) * rinv};
-            const int16_t* sl = ft.slot[idxp(l, m)];
+            // a = sum_n G dR_n/dt, b = sum_n G R_n
             const int nb = ft.nrl[l];
+            double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
 #pragma unroll
             for (int n = 0; n < NRM; ++n) {
                 if (n < nb) {
-                    const double2 G = STAGED ? Gs[(q + n) * MAXE] : Ge[(int64_t)sl[n] * estride];
-                    const double Rn = R[n], dRn = dR[n] * dtdr;
-                    const double gy = G.x * Yr - G.y * Yi;
-                    g0 += dRn * gy * rx + Rn * (G.x * dYr[0] - G.y * dYi[0]);
-                    g1 += dRn * gy * ry + Rn * (G.x * dYr[1] - G.y * dYi[1]);
-                    g2 += dRn * gy * rz + Rn * (G.x * dYr[2] - G.y * dYi[2]);
+                    const double2 G = Gs[(q + n) * MAXE];
+                    ax = fma(G.x, dR[n], ax);
+                    ay = fma(G.y, dR[n], ay);
+                    bx = fma(G.x, R[n], bx);
+                    by = fma(G.y, R[n], by);
                 }
             }
             q += nb;
+            force_lm_add(Pl, dPl, m, epr, epi, S, dtdr, rinv, rx, ry, rz, ax, ay, bx, by, g[0][0], g[0][1], g[0][2]);
+            force_keep<1>(g);
         }
     }
-    dE[j] = g0;
-    dE[nnbr + j] = g1;
-    dE[2 * nnbr + j] = g2;
+    dE[j] = g[0][0];
+    dE[nnbr + j] = g[0][1];
+    dE[2 * nnbr + j] = g[0][2];
 }
 
-template <int L, int NRM, int MAXE>
-__global__ __launch_bounds__(256, 4) void k_forcef(RadialParams rp, ForceFlatTab ft, int64_t nnbr,
+// WIDE = false: the blocks whose environments fit one round (the others
+// exit); WIDE = true: the blocks that need several rounds (the others exit).
+// Two kernels, so that the common one keeps the registers of a single round.
+// BS neighbours per block: 256, or 128 for short environments whose G table
+// only fits 16 environments (fewer environments per block, one round).
+template <int L, int NRM, int MAXE, bool WIDE, int BS>
+__global__ __launch_bounds__(BS, 4) void k_forcef(RadialParams rp, ForceFlatTab ft, int64_t nnbr,
                                                  const double* __restrict__ xyz,
                                                  const int32_t* __restrict__ nbr_env,
                                                  const int32_t* __restrict__ ffq,
                                                  const double2* __restrict__ Gbuf, int64_t estride,
                                                  double* __restrict__ dE) {
     extern __shared__ double2 Gst[];  // [q][MAXE]
-    const int64_t jb = (int64_t)blockIdx.x * 256;
+    const int64_t jb = (int64_t)blockIdx.x * BS;
     const int64_t j = jb + threadIdx.x;
-    const int64_t jl = min(jb + 255, nnbr - 1);
+    const int64_t jl = min(jb + BS - 1, nnbr - 1);
     // the block's environments (neighbours are grouped by environment)
     const int64_t e0 = nbr_env[jb];
     const int ne = (int)(nbr_env[jl] - e0 + 1);
-    if (ne <= MAXE) {
-        for (int i = threadIdx.x; i < ft.nq * ne; i += 256) {
-            const int qq = i / ne, el = i - qq * ne;
-            Gst[qq * MAXE + el] = Gbuf[(int64_t)ffq[qq] * estride + e0 + el];
+    if ((ne > MAXE) != WIDE) return;  // the other kernel's block: before any barrier
+    const int el = j < nnbr ? (int)(nbr_env[j] - e0) : -1;
+    for (int r0 = 0; r0 < ne; r0 += MAXE) {
+        const int nre = min(MAXE, ne - r0);
+        if (r0 > 0) __syncthreads();  // the previous round's reads are done
+        for (int i = threadIdx.x; i < ft.nq * nre; i += BS) {
+            const int qq = i / nre, ee = i - qq * nre;
+            Gst[qq * MAXE + ee] = Gbuf[(int64_t)ffq[qq] * estride + e0 + r0 + ee];
         }
         __syncthreads();
-        if (j >= nnbr) return;  // no barriers below
-        const int el = (int)(nbr_env[j] - e0);
-        forcef_lane<L, NRM, true, MAXE>(rp, ft, nnbr, j, xyz, nullptr, Gst + el, estride, dE);
-    } else {
-        if (j >= nnbr) return;
-        forcef_lane<L, NRM, false, MAXE>(rp, ft, nnbr, j, xyz, Gbuf + nbr_env[j], nullptr, estride, dE);
+        if (!WIDE) {
+            if (el < 0) return;  // no barriers below
+            forcef_lane<L, NRM, MAXE>(rp, ft, nnbr, j, xyz, Gst + el, dE);
+            return;
+        }
+        if (el >= r0 && el < r0 + nre) forcef_lane<L, NRM, MAXE>(rp, ft, nnbr, j, xyz, Gst + (el - r0), dE);
     }
 }
 
@@ -555,20 +547,32 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
         // blocks of short environments span more of them: stage 32 when that fits
         const bool wide = (size_t)ctx->fflat.nq * FF_MAXE_WIDE * sizeof(double2) <= FF_LDS;
         const size_t glds = (size_t)ctx->fflat.nq * (wide ? FF_MAXE_WIDE : FF_MAXE) * sizeof(double2);
-        const dim3 grid((unsigned)((nnbr + 255) / 256));
+        // short environments (fewer than 18 neighbours on average: a 256-neighbour
+        // block would span more than ~14) with a 16-environment stage:
+        // 128-neighbour blocks, so that a block's environments fit one round
+        const bool half = !wide && nnbr < (int64_t)18 * nenv;
         const int32_t* ne_ = ws.nbr_env;
+#define ACE_FORCEF(NRM_, MAXE_, BS_)                                                                           \
+    do {                                                                                                        \
+        const dim3 grid((unsigned)((nnbr + BS_ - 1) / BS_));                                                    \
+        hipLaunchKernelGGL((k_forcef<L, NRM_, MAXE_, false, BS_>), grid, dim3(BS_), glds, st,                   \
+                           radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);   \
+        hipLaunchKernelGGL((k_forcef<L, NRM_, MAXE_, true, BS_>), grid, dim3(BS_), glds, st,                    \
+                           radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);   \
+    } while (0)
         if (ctx->nr <= 12 && wide)
-            hipLaunchKernelGGL((k_forcef<L, 12, FF_MAXE_WIDE>), grid, dim3(256), glds, st, radial_params(ctx),
-                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
+            ACE_FORCEF(12, FF_MAXE_WIDE, 256);
+        else if (ctx->nr <= 12 && half)
+            ACE_FORCEF(12, FF_MAXE, 128);
         else if (ctx->nr <= 12)
-            hipLaunchKernelGGL((k_forcef<L, 12, FF_MAXE>), grid, dim3(256), glds, st, radial_params(ctx),
-                               ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
+            ACE_FORCEF(12, FF_MAXE, 256);
         else if (wide)
-            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX, FF_MAXE_WIDE>), grid, dim3(256), glds, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
+            ACE_FORCEF(FORCE_NRMAX, FF_MAXE_WIDE, 256);
+        else if (half)
+            ACE_FORCEF(FORCE_NRMAX, FF_MAXE, 128);
         else
-            hipLaunchKernelGGL((k_forcef<L, FORCE_NRMAX, FF_MAXE>), grid, dim3(256), glds, st,
-                               radial_params(ctx), ctx->fflat, nnbr, xyz, ne_, ctx->ffq.p, ws.Gbuf, estride, dE);
+            ACE_FORCEF(FORCE_NRMAX, FF_MAXE, 256);
+#undef ACE_FORCEF
         return;
     }
     const int nsp = std::max(1, ctx->nspecies);

@@ -105,8 +105,7 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
                         Pl = m == 0 ? S.cth * 1.7320508075688772 * 0.3989422804014327
                                     : S.cth * sqrt(2.0 * m + 3.0) * Pmm;
                     } else {
-                        const int i = idxp(l, m);
-                        Pl = rp.alpA[i] * (S.cth * Pb + rp.alpB[i] * Pa);
+                        Pl = alp_a(l, m) * (S.cth * Pb + alp_b(l, m) * Pa);
                     }
                     if (l > m) {
                         Pa = Pb;

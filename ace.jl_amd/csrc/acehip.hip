@@ -837,21 +837,6 @@ extern "C" int ace_abi_version(void) { return ACEHIP_ABI_VERSION; }
 
 extern "C" const char* ace_last_error(void) { return g_err.c_str(); }
 
-// ALP recursion coefficients (sphericalharmonics.jl:146-159): a_lm, then b_lm,
-// indexed by idxp(l, m); uploaded per context.
-static std::vector<double> alp_table() {
-    std::vector<double> t(2 * ACE_NPMAX, 0.0);
-    for (int l = 2; l <= ACE_LMAX; ++l) {
-        const double ls = (double)l * l, lm1s = (double)(l - 1) * (l - 1);
-        for (int m = 0; m <= l - 2; ++m) {
-            const double ms = (double)m * m;
-            t[idxp(l, m)] = std::sqrt((4 * ls - 1.0) / (ls - ms));
-            t[ACE_NPMAX + idxp(l, m)] = -std::sqrt((lm1s - ms) / (4 * lm1s - 1.0));
-        }
-    }
-    return t;
-}
-
 extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, int64_t nB,
                               ace_ctx** out) {
     if (!pk || !out || !c) return set_err("ace_ctx_create: null argument");
@@ -911,14 +896,12 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         delete ctx;
         return set_err("ace_ctx_create: hipGetSymbolAddress of the scratch array failed");
     }
-    const std::vector<double> alp = alp_table();
     if (hipHostMalloc((void**)&ctx->h_words, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
         ctx->h_words = nullptr;
         delete ctx;
         return set_err("ace_ctx_create: pinned host allocation failed");
     }
-    if (ctx->alp.upload(alp.data(), alp.size()) ||
-        ctx->rA.upload(pk->rA, pk->nr) || ctx->rB.upload(pk->rB, pk->nr) ||
+    if (ctx->rA.upload(pk->rA, pk->nr) || ctx->rB.upload(pk->rB, pk->nr) ||
         ctx->rC.upload(pk->rC, pk->nr) || ctx->pool.upload(pk->pool, 4 * (size_t)pk->npool) ||
         ctx->aspec_fid.upload(pk->aspec_fid, pk->nA) ||
         ctx->aspec_sgn.upload(pk->aspec_sgn, pk->nA) ||
@@ -977,6 +960,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             // for the prefix-shaped pooled sets of RnYlm bases); otherwise the
             // per-environment kernel runs
             std::vector<int> seen((size_t)(ACE_LMAX + 1) * (ACE_LMAX + 2) / 2 * FORCE_NRMAX, 0);
+            std::vector<int32_t> fslot(seen.size(), 0);  // G slot of (l, m >= 0, n)
             std::vector<int32_t> fold;
             for (int k = 0; k < pk->npool; ++k) {
                 const int n = pk->pool[4 * k + 1] - 1, l = pk->pool[4 * k + 2], m = pk->pool[4 * k + 3];
@@ -985,7 +969,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
                     ctx->flat_force = false;  // a pooled function without a gradient slot
                     break;
                 }
-                ctx->fflat.slot[idxp(l, m)][n] = (int16_t)pk->gslot_list[s0];
+                fslot[(size_t)idxp(l, m) * FORCE_NRMAX + n] = pk->gslot_list[s0];
                 seen[(size_t)idxp(l, m) * FORCE_NRMAX + n] = 1;
                 for (int q = s0 + 1; q < s1; ++q) {
                     fold.push_back(pk->gslot_list[s0]);
@@ -1001,7 +985,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             std::vector<int32_t> qslot;
             for (int m = 0; m <= pk->L && ctx->flat_force; ++m)
                 for (int l = m; l <= pk->L; ++l)
-                    for (int n = 0; n < ctx->nrl[l]; ++n) qslot.push_back(ctx->fflat.slot[idxp(l, m)][n]);
+                    for (int n = 0; n < ctx->nrl[l]; ++n) qslot.push_back(fslot[(size_t)idxp(l, m) * FORCE_NRMAX + n]);
             if ((int)qslot.size() > FF_QMAX) ctx->flat_force = false;
             ctx->fflat.nq = (int16_t)std::min<size_t>(qslot.size(), FF_QMAX);
             if (ctx->flat_force && ctx->ffq.upload(qslot.data(), qslot.size())) return fail("upload");

@@ -81,17 +81,19 @@ def test_segment_and_per_record_kernels(cuda, args, wid, nenv, kern):
         assert rel_err(dE.cpu().numpy().T, gref) < F_TOL
 
 
-@pytest.mark.parametrize("deg", [6, 12])
-def test_force_blocks_of_short_environments(cuda, deg):
-    """The flat force kernel stages the G of a 256-neighbour block's
-    environments in LDS (up to 32 of them for the small basis, 16 for the
-    order-4 one) and reads G from memory when a block spans more: one batch
-    holds blocks of both kinds."""
-    print(f"[gpu] force blocks of short environments deg={deg}", flush=True)
+@pytest.mark.parametrize("deg,nlong", [(6, 30), (12, 30), (12, 300)])
+def test_force_blocks_of_short_environments(cuda, deg, nlong):
+    """The flat force kernel stages the G of a block's environments in LDS (up
+    to 32 of them for the small basis, 16 for the order-4 one) and stages a
+    block that spans more in several rounds: one batch holds blocks of both
+    kinds.  150 environments of 1-3 neighbours then nlong of 20-40: with 30
+    long ones the order-4 basis runs 128-neighbour blocks (short on average),
+    with 300 256-neighbour blocks."""
+    print(f"[gpu] force blocks of short environments deg={deg} nlong={nlong}", flush=True)
     basis, T = basis_for(3, 6, 4) if deg == 6 else basis_for(4, 12, 6)
     c = synth.coefficients(len(basis), 3)
     o1, R1, _ = random_batch(150, 60, nmin=1, nmax=3)
-    o2, R2, _ = random_batch(30, 61, nmin=20, nmax=40)
+    o2, R2, _ = random_batch(nlong, 61, nmin=20, nmax=40)
     off = np.concatenate([o1, o2[1:] + o1[-1]]).astype(np.int32)
     R = np.concatenate([R1, R2])
     _, E, dE = _run(basis, T, c, off, R, waves=16 if deg == 12 else 8)
