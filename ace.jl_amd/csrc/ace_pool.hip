@@ -14,7 +14,7 @@
 // over the staged neighbours in order, holding the sums in registers across
 // chunks.
 template <int L, int EPW, int KP, int CH>
-__global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
+__global__ __launch_bounds__(64, KP > 4 ? 4 : 5) void k_pool(RadialParams rp, int64_t nenv,
                                               const int32_t* __restrict__ off,
                                               const double* __restrict__ xyz, int64_t nnbr,
                                               const int8_t* __restrict__ species,
@@ -119,16 +119,33 @@ __global__ __launch_bounds__(64) void k_pool(RadialParams rp, int64_t nenv,
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // lanes over pooled functions, neighbours summed in order
-        for (int jj = 0; jj < nch; ++jj) {
-            const double* slot = sgrp + jj * rstride;
-            const int smu = species ? (int)slot[nr + 2 * NP] : -1;
+        // lanes over pooled functions, neighbours summed in order.  One
+        // species: no branches, lanes past npool (kr = ky = 0) sum values they
+        // never store (KP is the smallest count of passes that covers npool).
+        if (species) {
+            // pooled functions are grouped by species, so most passes skip a
+            // neighbour of another species as a whole wave
+            for (int jj = 0; jj < nch; ++jj) {
+                const double* slot = sgrp + jj * rstride;
+                const int smu = (int)slot[nr + 2 * NP];
 #pragma unroll
-            for (int p = 0; p < KP; ++p) {
-                if (kmu[p] == -1 || kmu[p] == smu) {
+                for (int p = 0; p < KP; ++p) {
+                    if (kmu[p] == -1 || kmu[p] == smu) {
+                        const double R = slot[kr[p]];
+                        ar[p] = fma(R, slot[ky[p]], ar[p]);
+                        ai[p] = fma(R, slot[ky[p] + 1], ai[p]);
+                    }
+                }
+            }
+        } else {
+#pragma unroll 2
+            for (int jj = 0; jj < nch; ++jj) {
+                const double* slot = sgrp + jj * rstride;
+#pragma unroll
+                for (int p = 0; p < KP; ++p) {
                     const double R = slot[kr[p]];
-                    ar[p] += R * slot[ky[p]];
-                    ai[p] += R * slot[ky[p] + 1];
+                    ar[p] = fma(R, slot[ky[p]], ar[p]);
+                    ai[p] = fma(R, slot[ky[p] + 1], ai[p]);
                 }
             }
         }
@@ -306,9 +323,14 @@ static int launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t
     if (!ws.Aplus) return set_err("no buffer for the pooled values");
     const dim3 grid((unsigned)((nenv + epw - 1) / epw));
     const int8_t* sp = ctx->nspecies > 0 ? species : nullptr;
-    // pooled functions per lane: 1 (two environments per wave), 4 or 8
+    // pooled functions per lane: 1 (two environments per wave), else the
+    // fewest passes of 64 that cover npool: 2, 3, 4 or 8
     if (epw == 2)
         launch_pool_ch<L, 2, 1>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (ctx->npool <= 128)
+        launch_pool_ch<L, 1, 2>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (ctx->npool <= 192)
+        launch_pool_ch<L, 1, 3>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
     else if (ctx->npool <= 256)
         launch_pool_ch<L, 1, 4>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
     else if (ctx->npool <= 512)
