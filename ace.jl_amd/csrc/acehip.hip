@@ -511,17 +511,21 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __rest
 // self-contained records.  Every record recomputes its whole prefix product,
 // so the records of a task can be reordered: each task is a short list of
 // segments, each a run of records of one kind (prefix depth D = 0..7, NL =
-// 1..8 leaf slots), and the kernel switches once per segment, not once per
+// 1..8 leaves), and the kernel switches once per segment, not once per
 // record.  The loop over a segment has no branches, so the LDS reads of
 // consecutive records overlap.  Record words (flags, prefix offsets, leaf
 // offsets, padding to an even count, coefficients):
 //   w[0]              bit 31: the leaves are conjugated; bit 31 - t: prefix
-//                     factor t (1..3) is conjugated
+//                     factor t (1..7) is conjugated
 //   w[1 .. D]         LDS byte offsets of the prefix factors
-//   w[1 + D ..]       LDS byte offsets of the NL leaves (32-environment
-//                     tiles: NL even, the low half-wave takes the even leaf
-//                     slots and the high half the odd ones)
-//   then NL FP64 coefficients (CPLX: NL complex), zero for padding slots.
+//   w[1 + D ..]       LDS byte offsets of the NL leaves
+//   then NL FP64 coefficients (CPLX: NL complex).
+// 32-environment tiles (HALF): lanes e and e + 32 hold the same environment
+// and both run every record in full; the high half's results are dropped.
+// That duplicate work costs fewer instructions than splitting a record's
+// leaves between the halves, which needs a per-lane choice of every record
+// word (two vector instructions per 32-bit word, as an instruction reads one
+// scalar register): measured 16-22 % faster on 161-320 pooled values.
 template <int D, int NL, bool CPLX>
 struct SegShape {
     static constexpr int HW = (2 + D + NL) & ~1;           // header words, even
@@ -535,47 +539,37 @@ __device__ __forceinline__ double xor_sign(double c, uint32_t m) {
     return __hiloint2double(__double2hiint(c) ^ (int)(m & 0x80000000u), __double2loint(c));
 }
 
-template <int D, int NL, bool CPLX, bool HALF>
-__device__ __forceinline__ void dags_segment(const char* lds, const uint32_t* wp, int count, bool hi,
-                                             double2& acc) {
+template <int D, int NL, bool CPLX>
+__device__ __forceinline__ void dags_segment(const char* lds, const uint32_t* wp, int count, double2& acc) {
     using SH = SegShape<D, NL, CPLX>;
     constexpr int HW = SH::HW, RW = SH::RW;
-    constexpr int NV = HALF ? NL / 2 : NL;  // leaves per lane
     auto at = [&](uint32_t off) { return *reinterpret_cast<const double2*>(lds + off); };
     const SegWord* w = (const SegWord*)(uintptr_t)wp;
     // unrolled by 3 (real coefficients) or 2 (complex), by one less for
-    // records of more than 4 leaf slots (register use): measured fastest
+    // records of more than 4 leaves (register use): measured fastest
 #pragma unroll(CPLX ? (NL > 4 ? 1 : 2) : (NL > 4 ? 2 : 3))
     for (int i = 0; i < count; ++i, w += RW) {
         uint32_t h[RW];
 #pragma unroll
         for (int q = 0; q < RW; ++q) h[q] = w[q];
-        double2 v[NV], f[D > 0 ? D : 1];
+        double2 v[NL], f[D > 0 ? D : 1];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) v[q] = at(HALF ? pick(hi, h[1 + D + 2 * q], h[2 + D + 2 * q]) : h[1 + D + q]);
+        for (int q = 0; q < NL; ++q) v[q] = at(h[1 + D + q]);
 #pragma unroll
         for (int t = 0; t < D; ++t) f[t] = at(h[1 + t]);
         // S = sum of coefficient times leaf value: one fused multiply-add
         // chain per component
         double2 S;
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            const int z = HALF ? 2 * q : q;  // leaf slot (HALF: the low half's)
+        for (int q = 0; q < NL; ++q) {
             if (CPLX) {
-                const uint4 c = HALF ? make_uint4(pick(hi, h[HW + 4 * z], h[HW + 4 * z + 4]),
-                                                  pick(hi, h[HW + 4 * z + 1], h[HW + 4 * z + 5]),
-                                                  pick(hi, h[HW + 4 * z + 2], h[HW + 4 * z + 6]),
-                                                  pick(hi, h[HW + 4 * z + 3], h[HW + 4 * z + 7]))
-                                     : make_uint4(h[HW + 4 * z], h[HW + 4 * z + 1], h[HW + 4 * z + 2],
-                                                  h[HW + 4 * z + 3]);
+                const uint4 c = make_uint4(h[HW + 4 * q], h[HW + 4 * q + 1], h[HW + 4 * q + 2], h[HW + 4 * q + 3]);
                 if (q == 0)
                     rec_cfma<true>(S, c, v[q]);
                 else
                     rec_cfma<false>(S, c, v[q]);
             } else {
-                const double c = HALF ? rec_coef(pick(hi, h[HW + 2 * z], h[HW + 2 * z + 2]),
-                                                 pick(hi, h[HW + 2 * z + 1], h[HW + 2 * z + 3]))
-                                      : rec_coef(h[HW + 2 * z], h[HW + 2 * z + 1]);
+                const double c = rec_coef(h[HW + 2 * q], h[HW + 2 * q + 1]);
                 if (q == 0) {
                     S.x = c * v[q].x;
                     S.y = c * v[q].y;
@@ -658,33 +652,24 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
             const uint32_t* wp = segw + off;
 #define ACE_SEG(DD, NN)                                                                  \
     case DD * 8 + NN - 1:                                                                \
-        dags_segment<DD, NN, CPLX, HALF>(lds, wp, (int)count, hi, acc);                  \
+        dags_segment<DD, NN, CPLX>(lds, wp, (int)count, acc);                            \
         break;
 #define ACE_SEG4(DD) ACE_SEG(DD, 1) ACE_SEG(DD, 2) ACE_SEG(DD, 3) ACE_SEG(DD, 4)
 #define ACE_SEG8(DD) ACE_SEG4(DD) ACE_SEG(DD, 5) ACE_SEG(DD, 6) ACE_SEG(DD, 7) ACE_SEG(DD, 8)
-#define ACE_SEGH(DD) ACE_SEG(DD, 2) ACE_SEG(DD, 4) ACE_SEG(DD, 6) ACE_SEG(DD, 8)
-            // kind = depth * 8 + leaf slots - 1.  Records of up to 8 leaves
-            // (32-environment tiles: 2, 4, 6 or 8 slots); DEEP: prefixes of 4
-            // to 7 factors (correlation orders 5 to 8), records of up to 4
-            // leaves, 64-environment tiles only
+            // kind = depth * 8 + leaves - 1: records of up to 8 leaves; DEEP:
+            // prefixes of 4 to 7 factors (correlation orders 5 to 8), records
+            // of up to 4 leaves, 64-environment tiles only
             static_assert(!(HALF && DEEP), "deep prefixes run on 64-environment tiles");
-            if constexpr (HALF) {
-                switch (kind) { ACE_SEGH(0) ACE_SEGH(1) ACE_SEGH(2) ACE_SEGH(3) }
-            } else if constexpr (DEEP) {
+            if constexpr (DEEP) {
                 switch (kind) {
                     ACE_SEG4(0) ACE_SEG4(1) ACE_SEG4(2) ACE_SEG4(3) ACE_SEG4(4) ACE_SEG4(5) ACE_SEG4(6) ACE_SEG4(7)
                 }
             } else {
                 switch (kind) { ACE_SEG8(0) ACE_SEG8(1) ACE_SEG8(2) ACE_SEG8(3) }
             }
-#undef ACE_SEGH
 #undef ACE_SEG8
 #undef ACE_SEG4
 #undef ACE_SEG
-        }
-        if (HALF) {
-            acc.x += __shfl_xor(acc.x, 32);
-            acc.y += __shfl_xor(acc.y, 32);
         }
         if (slot < 0) {
             if (CPLX) {
@@ -814,8 +799,8 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
     }
     if (ctx->rec_et > 0) {
         acehost::Segments sg;
-        const std::string serr = acehost::segments(ctx->host, cf, ctx->rec_et == 64 ? 10 : 9, ctx->rec_et == 32,
-                                                   acehost::seg_leaves(ctx->maxorder), sg);
+        const std::string serr =
+            acehost::segments(ctx->host, cf, ctx->rec_et == 64 ? 10 : 9, acehost::seg_leaves(ctx->maxorder), sg);
         if (!serr.empty()) return set_err(serr);
         if (ctx->segw.upload(sg.words.data(), sg.words.size()) || ctx->segt.upload(sg.segt.data(), sg.segt.size()) ||
             ctx->seg_ptr.upload(sg.seg_ptr.data(), sg.seg_ptr.size()))

@@ -484,13 +484,11 @@ inline int task_records(const PackCopy& p, int t) {
 // The programs as kind-sorted segments of self-contained records (the
 // segment kernel).  Every run is cut into records of at most maxleaves
 // leaves; a record carries its whole prefix, so the records of a task can be
-// grouped by kind = depth * 8 + leaf slots - 1 (stable within a kind), one
+// grouped by kind = depth * 8 + leaves - 1 (stable within a kind), one
 // segment per kind present.  Record words: flags (bit 31 conjugated leaves,
 // bit 31 - t conjugated prefix factor t), the depth prefix offsets, the leaf
-// slot offsets, padding to an even count, then the slot coefficients (FP64,
-// or complex for class-split programs).  half: 32-environment tiles, leaf
-// slots rounded up to an even count (the two half-waves take alternate
-// slots); padding slots repeat leaf 0's offset with coefficient 0.
+// offsets, padding to an even count, then the leaf coefficients (FP64, or
+// complex for class-split programs).
 //   segt[4 * s]     kind, record count, word offset of the first record, 0
 //   seg_ptr[t]      the segments of task t: [seg_ptr[t], seg_ptr[t + 1])
 struct Segments {
@@ -498,8 +496,7 @@ struct Segments {
     std::vector<int32_t> seg_ptr;
 };
 
-inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, bool half, int maxleaves,
-                            Segments& out) {
+inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, int maxleaves, Segments& out) {
     if (maxleaves < 1 || maxleaves > 8) return "segment records hold 1 to 8 leaves";
     out.words.clear();
     out.segt.clear();
@@ -519,8 +516,7 @@ inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, 
             if (nl < 1) return "run without leaves";
             for (int q0 = 0; q0 < nl; q0 += maxleaves) {
                 const int n = std::min(maxleaves, nl - q0);
-                const int ns = half ? (n + 1) & ~1 : n;
-                by_kind[d * 8 + ns - 1].push_back({r, q0, n});
+                by_kind[d * 8 + n - 1].push_back({r, q0, n});
             }
         }
         for (int kind = 0; kind < NK; ++kind) {
@@ -543,7 +539,7 @@ inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, 
                     w[1 + q] = off(f);
                 }
                 w[0] = fl;
-                for (int i = 0; i < ns; ++i) w[1 + d + i] = off((uint32_t)p.leaf_fid[lo + (i < rc.n ? i : 0)]);
+                for (int i = 0; i < ns; ++i) w[1 + d + i] = off((uint32_t)p.leaf_fid[lo + i]);
                 for (int i = 0; i < rc.n; ++i) {
                     const size_t lf = (size_t)lo + i;
                     double c[2];

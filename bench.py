@@ -71,13 +71,12 @@ def executed_flops(pack, forces=True, kernel=0):
 
     Segment kernel (kernel code 0 / 1: 64- / 32-environment tiles), per record
     (a run's leaves cut into records of at most 8, 4 above correlation order
-    4) of depth d with s leaf slots: 4 per slot, less 2 for the first (a
+    4) of depth d with s leaves: 4 per leaf, less 2 for the first (a
     multiply), 8 for the complex coefficients of class-split EuclideanVector
     programs (less 4); the whole prefix product, 6 per factor after the first;
-    8 for the accumulation (2 without a prefix).  On 32-environment tiles the
-    slots are rounded up to an even count and both half-waves of an environment
-    compute the prefix and the accumulation.  Run interpreter: 4 per leaf, 6
-    per new prefix product, 8 per run."""
+    8 for the accumulation (2 without a prefix).  On 32-environment tiles both
+    half-waves of an environment run every record in full.  Run interpreter:
+    4 per leaf, 6 per new prefix product, 8 per run."""
     import numpy as np
     cplx = bool(getattr(pack, "classed", 0))
     if kernel not in (0, 1):
@@ -96,11 +95,9 @@ def executed_flops(pack, forces=True, kernel=0):
     q = np.arange(len(d)) - first                         # record index within its run
     nl = np.minimum(ch, np.repeat(rn, nrec) - ch * q)
     lanes = 2 if kernel == 1 else 1
-    slots = (nl + 1) // 2 * 2 if kernel == 1 else nl
-    per_slot = 8.0 if cplx else 4.0
-    leaf = per_slot * slots - per_slot / 2 * lanes
-    pre = 6.0 * np.maximum(d - 1, 0) + np.where(d > 0, 8.0, 2.0)
-    return float(np.sum(leaf + lanes * pre))
+    per_leaf = 8.0 if cplx else 4.0
+    rec = per_leaf * nl - per_leaf / 2 + 6.0 * np.maximum(d - 1, 0) + np.where(d > 0, 8.0, 2.0)
+    return float(lanes * np.sum(rec))
 
 
 def dag_kernel_name(code):
