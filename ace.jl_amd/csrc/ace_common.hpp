@@ -248,9 +248,12 @@ struct ace_ctx {
     // polynomial of class c = M + 1, G slots hold one complex value, and the
     // Cartesian components are sum_c u[c][p] (class value)
     int classed = 0;
-    // record kernel tile: 64 or 32 environments (0: the run interpreter only);
-    // force_interp: run the interpreter even so (ace_set_kernel)
+    // per-record kernel tile: 64 or 32 environments (0: no records);
+    // segment kernel tile: at most 64 environments, as many as fit the LDS
+    // (0: the run interpreter only); force_interp: run the interpreter even so
+    // (ace_set_kernel)
     int rec_et = 0;
+    int seg_et = 0;
     bool force_interp = false;
     // rec_dispatch: the record kernel that switches per record (ace_set_kernel
     // mode 2) instead of the segment kernel

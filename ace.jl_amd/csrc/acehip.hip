@@ -520,12 +520,12 @@ __global__ __launch_bounds__(1024) void k_dagr(DagRecArgs a, const uint4* __rest
 //   w[1 .. D]         LDS byte offsets of the prefix factors
 //   w[1 + D ..]       LDS byte offsets of the NL leaves
 //   then NL FP64 coefficients (CPLX: NL complex).
-// 32-environment tiles (HALF): lanes e and e + 32 hold the same environment
-// and both run every record in full; the high half's results are dropped.
-// That duplicate work costs fewer instructions than splitting a record's
-// leaves between the halves, which needs a per-lane choice of every record
-// word (two vector instructions per 32-bit word, as an instruction reads one
-// scalar register): measured 16-22 % faster on 161-320 pooled values.
+// Narrow tiles (more than 160 pooled values): ET < 64 environments, as
+// many as fit the LDS; the lanes past ET repeat lane 0's work and store
+// nothing.  Running whole records per lane costs fewer instructions than
+// splitting a record's leaves between lanes that hold the same environment,
+// which needs a per-lane choice of every record word (two vector instructions
+// per 32-bit word, as an instruction reads one scalar register).
 template <int D, int NL, bool CPLX>
 struct SegShape {
     static constexpr int HW = (2 + D + NL) & ~1;           // header words, even
@@ -598,19 +598,22 @@ __device__ __forceinline__ void dags_segment(const char* lds, const uint32_t* wp
     }
 }
 
-template <bool CPLX, bool HALF, bool DEEP>
+// ET environments per tile (ET <= 64, as many as the pooled values of which
+// fit the 160 KB of LDS): lane e < ET holds environment e; the lanes past ET
+// repeat lane 0's work and store nothing.
+template <bool CPLX, bool DEEP>
 __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __restrict__ segw,
                                                const uint4* __restrict__ segt, const int32_t* __restrict__ seg_ptr,
-                                               int64_t nenv, const double2* __restrict__ Aplus, int64_t estride,
-                                               double* __restrict__ E_out, double2* __restrict__ Gbuf,
-                                               const double* __restrict__ e0p, const int32_t* __restrict__ env_list,
+                                               int ET, int64_t nenv, const double2* __restrict__ Aplus,
+                                               int64_t estride, double* __restrict__ E_out,
+                                               double2* __restrict__ Gbuf, const double* __restrict__ e0p,
+                                               const int32_t* __restrict__ env_list,
                                                const int32_t* __restrict__ env_count) {
-    constexpr int ET = HALF ? 32 : 64;     // environments per tile
     constexpr int NE = CPLX ? 6 : 1;       // energy partials per lane
     extern __shared__ double2 As[];  // [npool][ET]
     const int lane = threadIdx.x & 63;
-    const int el = HALF ? (lane & 31) : lane;
-    const bool hi = HALF && lane >= 32;
+    const bool act = lane < ET;
+    const int el = act ? lane : 0;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t blo = a.ntile_lo * a.parts;
     const bool tail = (int64_t)blockIdx.x >= blo;
@@ -621,16 +624,15 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
     const int32_t* const wptr = tail ? a.wtask_ptr_hi : a.wtask_ptr;
     const int32_t* const wlist = tail ? a.wtask_list_hi : a.wtask_list;
     int64_t env = tile * ET + el;
-    bool valid = env < nenv;
+    bool valid = act && env < nenv;
     if (env_list) {
         const int64_t cnt = *env_count;
         if (tile * ET >= cnt) return;  // whole workgroup idle: before any barrier
-        valid = env < cnt;
+        valid = act && env < cnt;
         env = valid ? env_list[env] : 0;
     }
-    {
-        const int k0 = HALF ? 2 * wave + (hi ? 1 : 0) : wave, kstep = HALF ? 2 * a.W : a.W;
-        for (int k = k0; k < a.npool; k += kstep)
+    if (act) {
+        for (int k = wave; k < a.npool; k += a.W)
             As[k * ET + el] = valid ? nt_load2(Aplus + (int64_t)k * estride + env) : make_double2(0.0, 0.0);
     }
     __syncthreads();
@@ -644,7 +646,7 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
     for (int ti = tb; ti < te; ++ti) {
         const int t = wlist[ti];
         const int slot = a.tasks[5 * t + 2];
-        double2 acc = (HALF && hi) ? make_double2(0.0, 0.0) : make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
+        double2 acc = make_double2(a.task_const[2 * t], a.task_const[2 * t + 1]);
         const int sb = (int)sp[t], se = (int)sp[t + 1];
         for (int sg = sb; sg < se; ++sg) {
             const SegWord* st = (const SegWord*)(uintptr_t)(segt + sg);
@@ -658,8 +660,7 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
 #define ACE_SEG8(DD) ACE_SEG4(DD) ACE_SEG(DD, 5) ACE_SEG(DD, 6) ACE_SEG(DD, 7) ACE_SEG(DD, 8)
             // kind = depth * 8 + leaves - 1: records of up to 8 leaves; DEEP:
             // prefixes of 4 to 7 factors (correlation orders 5 to 8), records
-            // of up to 4 leaves, 64-environment tiles only
-            static_assert(!(HALF && DEEP), "deep prefixes run on 64-environment tiles");
+            // of up to 4 leaves
             if constexpr (DEEP) {
                 switch (kind) {
                     ACE_SEG4(0) ACE_SEG4(1) ACE_SEG4(2) ACE_SEG4(3) ACE_SEG4(4) ACE_SEG4(5) ACE_SEG4(6) ACE_SEG4(7)
@@ -683,18 +684,18 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
             } else {
                 ep[0] += acc.x;
             }
-        } else if (valid && !hi) {
+        } else if (valid) {
             nt_store2(Gbuf + (int64_t)slot * estride + env, acc);
         }
     }
     __syncthreads();
     double* red = reinterpret_cast<double*>(As);  // [W][NE][ET]
-    if (!hi) {
+    if (act) {
 #pragma unroll
         for (int q = 0; q < NE; ++q) red[(wave * NE + q) * ET + el] = ep[q];
     }
     __syncthreads();
-    if (wave == 0 && !hi && valid && E_out) {
+    if (wave == 0 && valid && E_out) {
         double Q[NE];
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
@@ -791,16 +792,15 @@ static int upload_coeffs(ace_ctx* ctx, const double* c) {
         }
         if (ctx->vleaf.upload(vl.data(), vl.size())) return -1;
     }
-    if (ctx->rec_et > 0 && ctx->nrec > 0) {
+    if (ctx->rec_et > 0) {
         std::vector<uint32_t> rec;
         const std::string rerr = acehost::records(ctx->host, cf, rec, ctx->rec_et == 64 ? 10 : 9);
         if (!rerr.empty()) return set_err(rerr);
         if (ctx->recs.upload(rec.data(), rec.size())) return -1;
     }
-    if (ctx->rec_et > 0) {
+    if (ctx->seg_et > 0) {
         acehost::Segments sg;
-        const std::string serr =
-            acehost::segments(ctx->host, cf, ctx->rec_et == 64 ? 10 : 9, acehost::seg_leaves(ctx->maxorder), sg);
+        const std::string serr = acehost::segments(ctx->host, cf, 16 * ctx->seg_et, acehost::seg_leaves(ctx->maxorder), sg);
         if (!serr.empty()) return set_err(serr);
         if (ctx->segw.upload(sg.words.data(), sg.words.size()) || ctx->segt.upload(sg.segt.data(), sg.segt.size()) ||
             ctx->seg_ptr.upload(sg.seg_ptr.data(), sg.seg_ptr.size()))
@@ -862,13 +862,20 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
     ctx->nrec = pk->nrec;
     ctx->ncoef = pk->prop == ACE_PROP_INVARIANT ? 1 : 6;
     ctx->classed = pk->classed;
-    // the record kernels' tile: 64 environments while their pooled values fit
-    // LDS (16 B each: npool <= 160), else 32 (npool <= 320; correlation order
-    // <= 4 only, deeper bases of that size run the interpreter)
-    if (pk->prop == ACE_PROP_INVARIANT || pk->classed)
-        ctx->rec_et = (size_t)pk->npool * 64 * sizeof(double2) <= 160 * 1024 ? 64
-                      : (size_t)pk->npool * 32 * sizeof(double2) <= 160 * 1024 && pk->maxorder <= 4 ? 32
-                                                                                                     : 0;
+    // the record kernels' tiles (Invariant and class-split programs): the
+    // segment kernel takes as many environments as fit the 160 KB of LDS (16 B
+    // per pooled value), at most 64 (npool <= 160), at least 20 (npool <= 512);
+    // deeper prefixes than 3 factors only on 64-environment tiles (else the
+    // interpreter).  The per-record kernel: 64 or 32 environments, for bases
+    // that carry records (correlation order <= 4, npool <= 320).
+    if (pk->prop == ACE_PROP_INVARIANT || pk->classed) {
+        const int et = (int)std::min<size_t>(64, (160 * 1024) / ((size_t)pk->npool * sizeof(double2)));
+        ctx->seg_et = pk->maxorder > 4 && et < 64 ? 0 : et;
+        if (pk->nrec > 0)
+            ctx->rec_et = (size_t)pk->npool * 64 * sizeof(double2) <= 160 * 1024   ? 64
+                          : (size_t)pk->npool * 32 * sizeof(double2) <= 160 * 1024 ? 32
+                                                                                   : 0;
+    }
     ctx->host = acehost::copy_pack(pk);
     const int nc = ctx->ncoef;
     auto fail = [&](const char* what) {
@@ -1003,7 +1010,7 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             wc->d_ptr_e.upload(wc->ptr_e.data(), wc->ptr_e.size()) ||
             wc->d_list_e.upload(wc->list_e.data(), wc->list_e.size()))
             return fail("upload");
-        if (ctx->rec_et > 0) {
+        if (ctx->seg_et > 0 || ctx->rec_et > 0) {
             const std::vector<int> t_all = acehost::mode_tasks(ctx->host, true), t_e = acehost::mode_tasks(ctx->host, false);
             for (int P = 2; P <= MAX_PARTS; ++P) {
                 std::vector<int32_t> ptr, list;
@@ -1274,27 +1281,22 @@ static int launch_dag_interp(ace_ctx* ctx, hipStream_t st, const DagArgs& a, int
                 : launch_dag_prop<1, 3>(ctx, st, a, nenv, estride, ws, E_out);
 }
 
-// The record kernel for a tile of ET (64 or 32) environments: the segment
-// kernel k_dags when ra carries segments, else k_dagr.
+// The record kernel for a tile of ET environments: the segment kernel k_dags
+// when ra carries segments (any ET <= 64), else k_dagr (ET 64 or 32).
 template <bool CPLX>
 static void launch_dagr(int ET, dim3 grid, dim3 block, size_t lds, hipStream_t st, const DagRecArgs& ra,
                         int64_t nenv, const double2* Aplus, int64_t estride, double* E_out, double2* Gbuf,
                         const double* e0p, const int32_t* env_list, const int32_t* env_count) {
     if (ra.segw) {
-#define ACE_DAGS(HALF_, DEEP_)                                                                              \
-    do {                                                                                                    \
-        set_func_lds((const void*)k_dags<CPLX, HALF_, DEEP_>, lds);                                         \
-        hipLaunchKernelGGL((k_dags<CPLX, HALF_, DEEP_>), grid, block, lds, st, ra, ra.segw, ra.segt,        \
-                           ra.seg_ptr, nenv, Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);        \
-    } while (0)
-        // deep prefixes only on 64-environment tiles (the context's choice)
-        if (ET == 64 && !ra.deep)
-            ACE_DAGS(false, false);
-        else if (ET == 64)
-            ACE_DAGS(false, true);
-        else
-            ACE_DAGS(true, false);
-#undef ACE_DAGS
+        if (!ra.deep) {
+            set_func_lds((const void*)k_dags<CPLX, false>, lds);
+            hipLaunchKernelGGL((k_dags<CPLX, false>), grid, block, lds, st, ra, ra.segw, ra.segt, ra.seg_ptr, ET, nenv,
+                               Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
+        } else {
+            set_func_lds((const void*)k_dags<CPLX, true>, lds);
+            hipLaunchKernelGGL((k_dags<CPLX, true>), grid, block, lds, st, ra, ra.segw, ra.segt, ra.seg_ptr, ET, nenv,
+                               Aplus, estride, E_out, Gbuf, e0p, env_list, env_count);
+        }
     } else if (ET == 64) {
         set_func_lds((const void*)k_dagr<CPLX, false>, lds);
         hipLaunchKernelGGL((k_dagr<CPLX, false>), grid, block, lds, st, ra, reinterpret_cast<const uint4*>(ra.recs),
@@ -1335,12 +1337,13 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
     // the record kernel: Invariant and class-split EuclideanVector bases of
     // order <= 4 with at most 320 pooled values; everything else runs the run
     // interpreter
-    const bool use_records = ctx->rec_et > 0 && !ctx->force_interp;
-    if (use_records && ctx->rec_dispatch && ctx->nrec == 0)
-        return set_err("ace_set_kernel mode 2: per-record dispatch needs correlation order <= 4");
+    const int rec_et = ctx->rec_dispatch ? ctx->rec_et : ctx->seg_et;
+    const bool use_records = rec_et > 0 && !ctx->force_interp;
+    if (ctx->rec_dispatch && ctx->rec_et == 0 && !ctx->force_interp)
+        return set_err("ace_set_kernel mode 2: per-record dispatch needs records (correlation order <= 4, npool <= 320)");
     ctx->last_tail = 0;
     if (use_records) {
-        const int ET = ctx->rec_et;
+        const int ET = rec_et;
         const int NE = ctx->classed ? 6 : 1;
         const int64_t ntile = (nenv + ET - 1) / ET;
         const size_t lds = std::max((size_t)ctx->npool * ET * sizeof(double2), (size_t)wc->W * NE * ET * sizeof(double));
@@ -1366,7 +1369,7 @@ static int eval_path(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t* 
             }
         }
         ctx->last_parts = P;
-        ctx->last_kernel = ET == 64 ? 0 : 1;
+        ctx->last_kernel = ET == 64 ? 0 : 1;  // 1: narrower tiles
         // tail tiles with P + 1 parts (P >= 2, so every tile sums shares in k_esum)
         int64_t nhi = 0;
         if (ctx->tail >= 0) {
@@ -1440,7 +1443,8 @@ static int run_path(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double
     const bool forces = dE != nullptr;
     if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
     if (forces && ctx->Gbuf.ensure((size_t)std::max(1, ctx->ngslots) * gslot_width(ctx) * estride)) return -1;
-    if (ctx->rec_et > 0 && ctx->Epart.ensure((size_t)MAX_PARTS * (ctx->classed ? 6 : 1) * estride)) return -1;
+    if ((ctx->seg_et > 0 || ctx->rec_et > 0) && ctx->Epart.ensure((size_t)MAX_PARTS * (ctx->classed ? 6 : 1) * estride))
+        return -1;
     if (forces && ctx->flat_force && nnbr > 0 && ctx->nbr_env.ensure(nnbr)) return -1;
     return eval_path(ctx, (hipStream_t)stream, nenv, off, xyz, nnbr, species, E_out, dE, ctx_buffers(ctx));
 }
@@ -1461,7 +1465,9 @@ extern "C" int ace_energy_forces_noalloc(ace_ctx* ctx, int64_t nenv, const int32
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t bA = up((size_t)ctx->npool * estride * sizeof(double2));
     const size_t bG = up((size_t)std::max(1, ctx->ngslots) * gslot_width(ctx) * estride * sizeof(double2));
-    const size_t bE = ctx->rec_et > 0 ? up((size_t)MAX_PARTS * (ctx->classed ? 6 : 1) * estride * sizeof(double)) : 0;
+    const size_t bE = ctx->seg_et > 0 || ctx->rec_et > 0
+                          ? up((size_t)MAX_PARTS * (ctx->classed ? 6 : 1) * estride * sizeof(double))
+                          : 0;
     const size_t bN = ctx->flat_force ? up((size_t)nnbr * sizeof(int32_t)) : 0;
     const size_t need = bA + bG + bE + bN;
     if (need > ACE_SCRATCH_BYTES)
@@ -1508,7 +1514,8 @@ int launch_dag_listed(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estrid
     if (ctx->prop != ACE_PROP_INVARIANT) return set_err("listed product-DAG evaluation: Invariant bases only");
     if (ctx->rec_dispatch && ctx->nrec == 0)
         return set_err("ace_set_kernel mode 2: per-record dispatch needs correlation order <= 4");
-    if (ctx->rec_et != 64) return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
+    if ((ctx->rec_dispatch ? ctx->rec_et : ctx->seg_et) != 64)
+        return set_err("pooled 1p basis too large for one LDS tile (npool > 160)");
     WaveCfg* wc = ctx->wcfg[ctx->cur_w];
     const int64_t ntile = (nenv + 63) / 64;
     DagRecArgs ra;

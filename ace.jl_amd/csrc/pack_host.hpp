@@ -496,13 +496,14 @@ struct Segments {
     std::vector<int32_t> seg_ptr;
 };
 
-inline std::string segments(const PackCopy& p, const Coeffs& cf, int lds_shift, int maxleaves, Segments& out) {
+// row_bytes: LDS bytes per pooled value (16 times the tile's environments).
+inline std::string segments(const PackCopy& p, const Coeffs& cf, int row_bytes, int maxleaves, Segments& out) {
     if (maxleaves < 1 || maxleaves > 8) return "segment records hold 1 to 8 leaves";
     out.words.clear();
     out.segt.clear();
     out.seg_ptr.assign(1, 0);
-    auto off = [&](uint32_t f) { return (f & 0x7fffu) << lds_shift; };
-    const int CW = p.classed ? 4 : 2;  // coefficient words per slot
+    auto off = [&](uint32_t f) { return (f & 0x7fffu) * (uint32_t)row_bytes; };
+    const int CW = p.classed ? 4 : 2;  // coefficient words per leaf
     constexpr int NK = 8 * (SEG_MAXD + 1);
     struct Rec {
         int run, q0, n;

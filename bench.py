@@ -74,8 +74,9 @@ def executed_flops(pack, forces=True, kernel=0):
     4) of depth d with s leaves: 4 per leaf, less 2 for the first (a
     multiply), 8 for the complex coefficients of class-split EuclideanVector
     programs (less 4); the whole prefix product, 6 per factor after the first;
-    8 for the accumulation (2 without a prefix).  On 32-environment tiles both
-    half-waves of an environment run every record in full.  Run interpreter:
+    8 for the accumulation (2 without a prefix).  On narrower tiles (ET < 64
+    environments, more than 160 pooled values) every lane of the wave runs
+    each record, so an environment costs 64 / ET lanes' work.  Run interpreter:
     4 per leaf, 6 per new prefix product, 8 per run."""
     import numpy as np
     cplx = bool(getattr(pack, "classed", 0))
@@ -94,7 +95,8 @@ def executed_flops(pack, forces=True, kernel=0):
     first = np.repeat(np.cumsum(nrec) - nrec, nrec)
     q = np.arange(len(d)) - first                         # record index within its run
     nl = np.minimum(ch, np.repeat(rn, nrec) - ch * q)
-    lanes = 2 if kernel == 1 else 1
+    et = min(64, (160 * 1024) // (16 * len(pack.pool)))   # the library's tile width
+    lanes = 64.0 / et if kernel == 1 else 1.0
     per_leaf = 8.0 if cplx else 4.0
     rec = per_leaf * nl - per_leaf / 2 + 6.0 * np.maximum(d - 1, 0) + np.where(d > 0, 8.0, 2.0)
     return float(lanes * np.sum(rec))
@@ -105,7 +107,7 @@ def dag_kernel_name(code):
     if code == 0:
         return "k_dags (kind-sorted record segments, 64-environment tiles)"
     if code == 1:
-        return "k_dags (kind-sorted record segments, 32-environment tiles, two half-waves per record)"
+        return "k_dags (kind-sorted record segments, tiles of fewer than 64 environments)"
     return f"k_dag run interpreter ({code}-environment tiles)"
 
 

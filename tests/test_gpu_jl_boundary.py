@@ -253,10 +253,11 @@ def test_lmax9_basis(cuda):
 @pytest.mark.parametrize("prop", ["Invariant", "EuclideanVector"])
 def test_three_species_beyond_320_pooled_values(cuda, prop):
     """Three species, order 2, degree 12, L 6: 417 (Invariant) and 498
-    (EuclideanVector, class-split) pooled values, beyond the 32-environment
-    record tiles: the run interpreter on 16-environment tiles (four task
-    streams per wave).  50 environments are four tiles; every energy and
-    gradient against the oracle, energy-only calls included, both compilers."""
+    (EuclideanVector, class-split) pooled values: the segment kernel on tiles
+    of 24 and 20 environments (as many as fit the LDS), and the run
+    interpreter (forced) on 16-environment tiles with four task streams per
+    wave.  50 environments are several tiles; every energy and gradient
+    against the oracle, energy-only calls included, both compilers."""
     print(f"[gpu] three species, npool > 320, {prop}", flush=True)
     basis, T = basis_for(2, 12, 6, species=3, prop=prop)
     c = synth.coefficients(len(basis), 35)
@@ -264,12 +265,13 @@ def test_three_species_beyond_320_pooled_values(cuda, prop):
     b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
     E_ref = O.energy(T, c, off, R, sp)
     g_ref = O.grad_config(T, c, off, R, sp)
-    for compiler in ("native", "python"):
+    for compiler, mode, kern in (("native", 0, 1), ("native", 1, 16), ("python", 0, 1)):
         ev = HIPEvaluator(basis, c, device=0, waves=8, compiler=compiler)
+        ev.set_kernel(mode)
         E, dE = ev.energy_forces(b)
         E1 = ev.energy(b)
         torch.cuda.synchronize()
-        assert ev.last_dag_shape()[2] == 16
+        assert ev.last_dag_shape()[2] == kern
         E, dE, E1 = E.cpu().numpy(), dE.cpu().numpy(), E1.cpu().numpy()
         assert rel_err(E, E_ref) < E_TOL
         assert rel_err(E1, E_ref) < E_TOL
