@@ -31,6 +31,14 @@
 #define ACE_CONJ 0x8000
 
 __host__ __device__ constexpr int idxp(int l, int m) { return m + l * (l + 1) / 2; }
+// k_pool's neighbour slot (doubles): R_n from 0, Y_lm (NP complex values) from
+// the even offset pool_yoff, then the species; an even count in all, so that
+// every Y_lm is a 16-byte aligned LDS read
+__host__ __device__ constexpr int pool_yoff(int nr) { return (nr + 1) & ~1; }
+__host__ __device__ constexpr int pool_rstride(int nr, int np) { return (pool_yoff(nr) + 2 * np + 2) & ~1; }
+// k_pool's pooled functions per lane with one environment per wave: the
+// fewest passes of 64 lanes that cover npool among 2, 3, 4 and 8
+inline int pool_passes(int npool) { return npool <= 128 ? 2 : npool <= 192 ? 3 : npool <= 256 ? 4 : 8; }
 
 // Per-neighbour parameters of the 1p basis: the radial recursion
 // coefficients and envelope (orthpolys.jl:82-99, distancetransforms.jl:73).
@@ -261,6 +269,12 @@ struct ace_ctx {
     // pooling launch shape (ace_set_pool_shape): neighbours per chunk (8, 16,
     // 32; 0 = 16) and environments per wave (0 = automatic, 1 = one)
     int pool_ch = 0, pool_epw = 0;
+    // shared-Y lane sets of k_pool: pool_slot[hl + 64 p] is the pooled function
+    // of lane hl in pass p (-1: none); the KP functions of a lane share
+    // (species, l, m).  pool_shy_kp: the pass count the table is laid out for
+    // (pool_passes(npool)), 0 when the sets need more than 64 lanes
+    int pool_shy_kp = 0;
+    DevBuf<int32_t> pool_slot;
     // basis-gradient weight tables: built on the host at context creation,
     // uploaded at the first ace_basis_dB call; db_usable: they fit the
     // weight kernel's LDS budget (DB_LDS_BUDGET)

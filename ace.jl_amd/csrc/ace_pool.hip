@@ -9,46 +9,59 @@
 // k_pool: A+[k][env].  EPW environments per wave, one per group of
 // LPE = 64 / EPW lanes (EPW = 2 possible when npool <= 32, so that small
 // bases keep the lanes busy).  A group stages its environment's neighbours CH
-// at a time (CH <= LPE) in its own LDS region ([slot][R_n | Y_lm | species]);
-// its lanes then turn into pooled functions k = hl + LPE p (p < KP) and sum
-// over the staged neighbours in order, holding the sums in registers across
-// chunks.
-template <int L, int EPW, int KP, int CH>
+// at a time (CH <= LPE) in its own LDS region ([slot][R_n | pad | Y_lm |
+// species], Y at an even offset so that one 16-byte read fetches Y_lm); its
+// lanes then turn into pooled functions and sum over the staged neighbours in
+// order, holding the sums in registers across chunks.  The summation reads
+// LDS once per function for R_n and once per Y_lm it needs: SHY (shared Y),
+// the lane's KP functions k = pslot[hl + 64 p] share (species, l, m), so Y_lm
+// is read once per neighbour (16 bytes instead of 16 KP; the launcher takes
+// it for bases without species, whose summation has no per-function branch);
+// otherwise lane hl takes k = hl + LPE p.
+template <int L, int EPW, int KP, int CH, bool SHY>
 __global__ __launch_bounds__(64, KP > 4 ? 4 : 5) void k_pool(RadialParams rp, int64_t nenv,
                                               const int32_t* __restrict__ off,
                                               const double* __restrict__ xyz, int64_t nnbr,
                                               const int8_t* __restrict__ species,
-                                              const int32_t* __restrict__ pool, int npool,
-                                              double2* __restrict__ Aplus, int64_t estride,
+                                              const int32_t* __restrict__ pool, const int32_t* __restrict__ pslot,
+                                              int npool, double2* __restrict__ Aplus, int64_t estride,
                                               int32_t* __restrict__ nbr_env) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
     constexpr int LPE = 64 / EPW;                 // lanes per environment
     static_assert(EPW == 1 || KP == 1, "two environments per wave take one pooled function per lane");
     static_assert(CH <= LPE, "a chunk holds at most one neighbour per lane of the group");
+    static_assert(!SHY || EPW == 1, "shared-Y lane tables are laid out for one environment per wave");
     extern __shared__ double smem[];  // [EPW][CH][rstride]
     const int lane = threadIdx.x;
     const int hl = lane % LPE;
     const int nr = rp.nr;
-    const int rstride = nr + 2 * NP + 1;  // doubles per neighbour slot
+    const int yoff = pool_yoff(nr);
+    const int rstride = pool_rstride(nr, NP);  // doubles per neighbour slot
     const int64_t nblk = (nenv + EPW - 1) / EPW;
     const int64_t env = xcd_env(blockIdx.x, nblk) * EPW + lane / LPE;
     const bool valid = env < nenv;
     const int j0 = valid ? off[env] : 0, j1 = valid ? off[env + 1] : 0;
     double* const sgrp = smem + (lane / LPE) * CH * rstride;
-    // this lane's pooled functions k = hl + LPE p: slot offsets of R_n and Y_lm
+    // this lane's pooled functions: index, species, slot offsets of R_n and Y_lm
     int kr[KP], ky[KP], kmu[KP];
     double ar[KP], ai[KP];
 #pragma unroll
     for (int p = 0; p < KP; ++p) {
-        const int k = hl + LPE * p;
-        kr[p] = ky[p] = 0;
+        const int k = SHY ? pslot[hl + 64 * p] : (hl + LPE * p < npool ? hl + LPE * p : -1);
+        kr[p] = 0;
+        ky[p] = yoff;
         kmu[p] = -2;  // -2: no pooled function in this pass
-        if (k < npool) {
+        if (k >= 0) {
             kmu[p] = pool[4 * k];
             kr[p] = pool[4 * k + 1] - 1;
-            ky[p] = nr + 2 * idxp(pool[4 * k + 2], pool[4 * k + 3]);
+            ky[p] = yoff + 2 * idxp(pool[4 * k + 2], pool[4 * k + 3]);
         }
         ar[p] = ai[p] = 0.0;
+    }
+    if (SHY) {
+        // a set's first member is valid whenever the lane has one
+#pragma unroll
+        for (int p = 1; p < KP; ++p) ky[p] = ky[0];
     }
     for (int jb = j0; jb < j1; jb += CH) {
         const int nch = min(CH, j1 - jb);
@@ -78,7 +91,7 @@ __global__ __launch_bounds__(64, KP > 4 ? 4 : 5) void k_pool(RadialParams rp, in
             // Y_lm = e^{i m phi}/sqrt2 P_lm, m-outer: P_mm from P_{m-1,m-1}, then
             // the l recursion at fixed m (the operations of alp_eval, in an
             // order that keeps two degrees live), written straight to LDS
-            double* ys = slot + nr;
+            double* ys = slot + yoff;
             const double isq2 = 0.7071067811865475;
             double epr = isq2, epi = 0.0;
             double Pmm = 0.3989422804014327;  // sqrt(1/(2 pi))
@@ -115,37 +128,41 @@ __global__ __launch_bounds__(64, KP > 4 ? 4 : 5) void k_pool(RadialParams rp, in
                     ys[2 * idxp(l, m) + 1] = m == 0 ? 0.0 : epi * Pl;
                 }
             }
-            slot[nr + 2 * NP] = species ? (double)species[j] : -1.0;
+            slot[yoff + 2 * NP] = species ? (double)species[j] : -1.0;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // lanes over pooled functions, neighbours summed in order.  One
-        // species: no branches, lanes past npool (kr = ky = 0) sum values they
-        // never store (KP is the smallest count of passes that covers npool).
+        // species: no branches, lanes past npool (kr = 0, ky = yoff) sum values
+        // they never store (KP is the smallest count of passes that covers npool).
         if (species) {
             // pooled functions are grouped by species, so most passes skip a
             // neighbour of another species as a whole wave
             for (int jj = 0; jj < nch; ++jj) {
                 const double* slot = sgrp + jj * rstride;
-                const int smu = (int)slot[nr + 2 * NP];
+                const int smu = (int)slot[yoff + 2 * NP];
 #pragma unroll
                 for (int p = 0; p < KP; ++p) {
                     if (kmu[p] == -1 || kmu[p] == smu) {
                         const double R = slot[kr[p]];
-                        ar[p] = fma(R, slot[ky[p]], ar[p]);
-                        ai[p] = fma(R, slot[ky[p] + 1], ai[p]);
+                        const double2 Y = *reinterpret_cast<const double2*>(slot + ky[p]);
+                        ar[p] = fma(R, Y.x, ar[p]);
+                        ai[p] = fma(R, Y.y, ai[p]);
                     }
                 }
             }
         } else {
-#pragma unroll 2
+            // two neighbours per iteration, one for 8 passes (registers)
+#pragma unroll(KP > 4 ? 1 : 2)
             for (int jj = 0; jj < nch; ++jj) {
                 const double* slot = sgrp + jj * rstride;
+                double2 Y = *reinterpret_cast<const double2*>(slot + ky[0]);
 #pragma unroll
                 for (int p = 0; p < KP; ++p) {
+                    if (!SHY && p > 0) Y = *reinterpret_cast<const double2*>(slot + ky[p]);
                     const double R = slot[kr[p]];
-                    ar[p] = fma(R, slot[ky[p]], ar[p]);
-                    ai[p] = fma(R, slot[ky[p] + 1], ai[p]);
+                    ar[p] = fma(R, Y.x, ar[p]);
+                    ai[p] = fma(R, Y.y, ai[p]);
                 }
             }
         }
@@ -155,7 +172,10 @@ __global__ __launch_bounds__(64, KP > 4 ? 4 : 5) void k_pool(RadialParams rp, in
     if (!valid) return;
 #pragma unroll
     for (int p = 0; p < KP; ++p)
-        if (kmu[p] != -2) Aplus[(int64_t)(hl + LPE * p) * estride + env] = make_double2(ar[p], ai[p]);
+        if (kmu[p] != -2) {
+            const int k = SHY ? pslot[hl + 64 * p] : hl + LPE * p;
+            Aplus[(int64_t)k * estride + env] = make_double2(ar[p], ai[p]);
+        }
 }
 
 
@@ -283,25 +303,36 @@ __global__ __launch_bounds__(64) void k_poolw(RadialParams rp, int64_t nenv, con
         if (kmu[p] != -2) dAw[(int64_t)(lane + 64 * p) * estride + env] = make_double2(ar[p], ai[p]);
 }
 
-template <int L, int EPW, int KP, int CH>
+template <int L, int EPW, int KP, int CH, bool SHY>
 static void launch_pool_shape(const dim3& grid, size_t lds, hipStream_t st, ace_ctx* ctx, int64_t nenv,
                               const int32_t* off, const double* xyz, int64_t nnbr, const int8_t* sp,
                               const Workspace& ws, int64_t estride, int32_t* nbr_env) {
-    set_func_lds((const void*)k_pool<L, EPW, KP, CH>, lds);
-    hipLaunchKernelGGL((k_pool<L, EPW, KP, CH>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz, nnbr,
-                       sp, ctx->pool.p, ctx->npool, ws.Aplus, estride, nbr_env);
+    set_func_lds((const void*)k_pool<L, EPW, KP, CH, SHY>, lds);
+    hipLaunchKernelGGL((k_pool<L, EPW, KP, CH, SHY>), grid, dim3(64), lds, st, radial_params(ctx), nenv, off, xyz,
+                       nnbr, sp, ctx->pool.p, SHY ? ctx->pool_slot.p : nullptr, ctx->npool, ws.Aplus, estride,
+                       nbr_env);
 }
 
-template <int L, int EPW, int KP>
+template <int L, int EPW, int KP, bool SHY>
 static void launch_pool_ch(int ch, const dim3& grid, size_t lds, hipStream_t st, ace_ctx* ctx, int64_t nenv,
                            const int32_t* off, const double* xyz, int64_t nnbr, const int8_t* sp, const Workspace& ws,
                            int64_t estride, int32_t* nbr_env) {
     if (ch == 8)
-        launch_pool_shape<L, EPW, KP, 8>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+        launch_pool_shape<L, EPW, KP, 8, SHY>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
     else if (ch == 32)
-        launch_pool_shape<L, EPW, KP, 32>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+        launch_pool_shape<L, EPW, KP, 32, SHY>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
     else
-        launch_pool_shape<L, EPW, KP, 16>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+        launch_pool_shape<L, EPW, KP, 16, SHY>(grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+}
+
+template <int L, int KP>
+static void launch_pool_kp(bool shy, int ch, const dim3& grid, size_t lds, hipStream_t st, ace_ctx* ctx, int64_t nenv,
+                           const int32_t* off, const double* xyz, int64_t nnbr, const int8_t* sp, const Workspace& ws,
+                           int64_t estride, int32_t* nbr_env) {
+    if (shy)
+        launch_pool_ch<L, 1, KP, true>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else
+        launch_pool_ch<L, 1, KP, false>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
 }
 
 template <int L>
@@ -314,29 +345,31 @@ static int launch_pool(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32_t
     // context's choice, 16 by default
     const int epw = ctx->npool <= 32 && ctx->pool_epw != 1 ? 2 : 1;
     const int ch = ctx->pool_ch > 0 ? ctx->pool_ch : POOL_CH;
-    const size_t lds = (size_t)epw * ch * (ctx->nr + 2 * NP + 1) * sizeof(double);
+    const size_t lds = (size_t)epw * ch * pool_rstride(ctx->nr, NP) * sizeof(double);
     int32_t* nbr_env = nullptr;
     if (nbr_map && nnbr > 0) {
         if (!ws.nbr_env || ws.nbr_cap < (size_t)nnbr) return set_err("neighbour map buffer too small");
         nbr_env = ws.nbr_env;
     }
     if (!ws.Aplus) return set_err("no buffer for the pooled values");
+    if (ctx->npool > 512) return set_err("pooled 1p basis too large (npool > 512)");
     const dim3 grid((unsigned)((nenv + epw - 1) / epw));
     const int8_t* sp = ctx->nspecies > 0 ? species : nullptr;
     // pooled functions per lane: 1 (two environments per wave), else the
-    // fewest passes of 64 that cover npool: 2, 3, 4 or 8
+    // fewest passes of 64 that cover npool: 2, 3, 4 or 8; the shared-Y lane
+    // sets when the basis has no species and the sets fit the wave
+    const int kp = pool_passes(ctx->npool);
+    const bool shy = !sp && ctx->pool_shy_kp == kp;
     if (epw == 2)
-        launch_pool_ch<L, 2, 1>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
-    else if (ctx->npool <= 128)
-        launch_pool_ch<L, 1, 2>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
-    else if (ctx->npool <= 192)
-        launch_pool_ch<L, 1, 3>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
-    else if (ctx->npool <= 256)
-        launch_pool_ch<L, 1, 4>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
-    else if (ctx->npool <= 512)
-        launch_pool_ch<L, 1, 8>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+        launch_pool_ch<L, 2, 1, false>(ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (kp == 2)
+        launch_pool_kp<L, 2>(shy, ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (kp == 3)
+        launch_pool_kp<L, 3>(shy, ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
+    else if (kp == 4)
+        launch_pool_kp<L, 4>(shy, ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
     else
-        return set_err("pooled 1p basis too large (npool > 512)");
+        launch_pool_kp<L, 8>(shy, ch, grid, lds, st, ctx, nenv, off, xyz, nnbr, sp, ws, estride, nbr_env);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -941,6 +941,12 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
         if (ctx->aspec4.upload(a4.data(), a4.size())) return fail("upload");
         std::vector<int16_t> kx = acehost::pool_lookup(pk);
         if (ctx->kidx.upload(kx.data(), kx.size())) return fail("upload");
+        // the pooling kernel's shared-Y lane sets at its pass count
+        std::vector<int32_t> pslot;
+        if (acehost::pool_lane_sets(pk->pool, pk->npool, pool_passes(pk->npool), pslot)) {
+            if (ctx->pool_slot.upload(pslot.data(), pslot.size())) return fail("upload");
+            ctx->pool_shy_kp = pool_passes(pk->npool);
+        }
         for (int k = 0; k < pk->npool; ++k)
             ctx->nrl[pk->pool[4 * k + 2]] = std::max(ctx->nrl[pk->pool[4 * k + 2]], (int)pk->pool[4 * k + 1]);
         ctx->flat_force = pk->prop == ACE_PROP_INVARIANT && pk->nspecies <= 1 && pk->nr <= FORCE_NRMAX &&

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/acehip.h"
@@ -682,6 +683,32 @@ inline std::vector<int32_t> aspec_rows(const ace_pack* pk) {
         a4[4 * i + 3] = (f & 0x8000) ? -pk->pool[4 * k + 3] : pk->pool[4 * k + 3];
     }
     return a4;
+}
+
+// Lane sets of the pooling kernel for kp pooled functions per lane: the
+// pooled functions grouped by (species, l, m), each group cut into sets of at
+// most kp, set g on lane g; slot[g + 64 p] is the set's p-th member (-1:
+// none).  The kp functions of a lane then share one Y_lm.  Returns false (and
+// leaves slot empty) when the sets need more than 64 lanes.
+inline bool pool_lane_sets(const int32_t* pool, int npool, int kp, std::vector<int32_t>& slot) {
+    slot.clear();
+    if (kp < 1 || npool < 1) return false;
+    std::vector<int> order(npool);
+    for (int k = 0; k < npool; ++k) order[k] = k;
+    auto key = [&](int k) { return std::make_tuple(pool[4 * k], pool[4 * k + 2], pool[4 * k + 3]); };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+    std::vector<int32_t> out(64 * (size_t)kp, -1);
+    int g = -1, fill = kp;
+    for (int i = 0; i < npool; ++i) {
+        const int k = order[i];
+        if (fill == kp || key(k) != key(order[i - 1])) {
+            if (++g >= 64) return false;
+            fill = 0;
+        }
+        out[g + 64 * (size_t)fill++] = k;
+    }
+    slot.swap(out);
+    return true;
 }
 
 }  // namespace acehost

@@ -39,6 +39,15 @@ extern "C" int acehost_lookup(const ace_pack* pk, short* kidx, int* a4) {
     return 0;
 }
 
+// Lane sets of the pooling kernel: writes 64 kp slots, returns 1 when the
+// sets fit 64 lanes, else 0.
+extern "C" int acehost_pool_sets(const int32_t* pool, int npool, int kp, int32_t* slot) {
+    std::vector<int32_t> v;
+    if (!acehost::pool_lane_sets(pool, npool, kp, v)) return 0;
+    std::memcpy(slot, v.data(), v.size() * sizeof(int32_t));
+    return 1;
+}
+
 // Neighbour-list binning (nbrlist_host.hpp).  bin_start must hold
 // 2 * natoms + 66 entries (the bin count never exceeds 2 natoms + 64).
 extern "C" int acehost_nbr_grid(int64_t natoms, const double* pos, const double* cell, const int32_t* pbc,

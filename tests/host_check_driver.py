@@ -58,6 +58,25 @@ for args in [(3, 6, 4, "Invariant", 0), (3, 6, 4, "EuclideanVector", 0), (3, 6, 
     a4 = np.zeros(4 * len(pk.aspec_fid), dtype=np.int32)
     assert lib.acehost_lookup(C.byref(s), kidx.ctypes.data_as(C.c_void_p), a4.ctypes.data_as(C.c_void_p)) == 0
     assert np.all(a4.reshape(-1, 4) == np.asarray(T["aspec"]))
+    # pooling lane sets: every pooled function on exactly one (lane, pass);
+    # the functions of a lane share (species, l, m); sets fit 64 lanes or none
+    P = np.ascontiguousarray(pk.pool, dtype=np.int32).reshape(-1, 4)
+    for kp in (1, 2, 3, 8):
+        slot = np.full(64 * kp, -7, dtype=np.int32)
+        ok = lib.acehost_pool_sets(P.ctypes.data_as(C.c_void_p), len(P), kp, slot.ctypes.data_as(C.c_void_p))
+        cls = {}
+        for mu, n, l, m in P.tolist():
+            cls[(mu, l, m)] = cls.get((mu, l, m), 0) + 1
+        need = sum(-(-v // kp) for v in cls.values())
+        assert ok == (need <= 64), (kp, ok, need)
+        if ok:
+            S = slot.reshape(kp, 64)
+            assert sorted(S[S >= 0].tolist()) == list(range(len(P)))
+            for g in range(64):
+                ks = S[:, g][S[:, g] >= 0]
+                assert len({(P[k, 0], P[k, 2], P[k, 3]) for k in ks}) <= 1
+                # members fill the first passes of a lane
+                assert np.all(S[:len(ks), g] >= 0)
     # corrupted packs are rejected with a message, never read out of bounds
     for field, bad, expect in [("npool", 0, b"npool"), ("L", 11, b"L must"), ("maxorder", 9, b"maxorder"),
                                ("ngslots", 0, b"task out of range")]:
