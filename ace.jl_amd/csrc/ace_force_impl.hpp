@@ -493,7 +493,9 @@ __global__ __launch_bounds__(BS, 4) void k_forcef(RadialParams rp, ForceFlatTab 
                                                  const double2* __restrict__ Gbuf, int64_t estride,
                                                  double* __restrict__ dE) {
     extern __shared__ double2 Gst[];  // [q][MAXE]
-    const int64_t jb = (int64_t)blockIdx.x * BS;
+    // blocks XCD-contiguous: neighbouring blocks, which stage G rows of the
+    // same cache lines (adjacent environments), share one L2
+    const int64_t jb = xcd_env(blockIdx.x, gridDim.x) * BS;
     const int64_t j = jb + threadIdx.x;
     const int64_t jl = min(jb + BS - 1, nnbr - 1);
     // the block's environments (neighbours are grouped by environment)
