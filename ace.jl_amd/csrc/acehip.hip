@@ -618,7 +618,11 @@ __global__ __launch_bounds__(1024) void k_dags(DagRecArgs a, const uint32_t* __r
     const int64_t blo = a.ntile_lo * a.parts;
     const bool tail = (int64_t)blockIdx.x >= blo;
     const int P = tail ? a.parts + 1 : a.parts;
-    const int64_t bt = tail ? (int64_t)blockIdx.x - blo : (int64_t)blockIdx.x;
+    // the parts of a tile on one XCD (the first tiles' blocks XCD-contiguous),
+    // so that the A+ tile they each load meets in one L2; not for a listed
+    // subset, whose working tiles are a prefix (XCD-contiguous blocks would
+    // leave the last XCDs idle)
+    const int64_t bt = tail ? (int64_t)blockIdx.x - blo : env_list ? (int64_t)blockIdx.x : xcd_env(blockIdx.x, blo);
     const int64_t tile = (tail ? a.ntile_lo : 0) + bt / P;
     const int part = (int)(bt % P);
     const int32_t* const wptr = tail ? a.wtask_ptr_hi : a.wtask_ptr;

@@ -26,10 +26,11 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per product-DAG launch on this workload (100k environments, 16-wave
 # tiles split over 3 workgroups), from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs of this script, scaled by the factors a 256 MiB device copy
-# gives the same counters (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE): 715.6 MB
-# read + 269.7 MB written.  Only valid for the default --nenv; re-measured
-# whenever the kernel changes.
-DAG_TRAFFIC_BYTES = {100_000: 985.3e6}
+# gives the same counters (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE): 418.2 MB
+# read (the three workgroups of a tile each load its A+ rows, mostly from the
+# XCD's L2 after the first) + 269.7 MB written.  Only valid for the default
+# --nenv; re-measured whenever the kernel changes.
+DAG_TRAFFIC_BYTES = {100_000: 687.9e6}
 WORKLOAD_NAMES = {
     1: "plain ACE energy+forces, 1000 envs x 12 neighbours, order 3, degree 6, Lmax 4, Invariant, FP64",
     2: "plain ACE energy+forces, 100k envs, nX~Poisson(30) clipped [5,80], order 4, degree 12, Lmax 6, "
