@@ -10,6 +10,7 @@ import torch
 import ace_oracle as O
 from acehip import synth
 from acehip.evaluator import EnvBatch, HIPEvaluator
+from acehip.pack import Pack
 from helpers import basis_for, random_batch, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -431,6 +432,38 @@ def test_pool_shapes_bit_identical(cuda, args, wid, nenv):
         assert torch.equal(E, E0) and torch.equal(dE, dE0), (ch, epw)
     n = 16
     assert rel_err(E0.cpu().numpy()[:n], O.energy(T, c, off[: n + 1], R[: off[n]])) < E_TOL
+
+
+
+@pytest.mark.parametrize("args,shared", [((2, 18, 6), True), ((2, 16, 6), False)],
+                         ids=["npool307-shared-Y", "npool251-per-pass-Y"])
+def test_pool_lane_sets_large_bases(cuda, args, shared):
+    """One species, more than 192 pooled values: 307 pooled functions take 8
+    passes whose lane sets (functions of one (l, m) per lane) fit 64 lanes, so
+    each lane reads Y once per neighbour; 251 take 4 passes whose sets need 77
+    lanes, so lanes read Y per pass.  Energies and forces match the oracle and
+    every chunk width gives bit-identical results."""
+    print(f"[gpu] pooling lane sets {args}", flush=True)
+    basis, T = basis_for(*args)
+    P = np.asarray(Pack(T).pool).reshape(-1, 4)
+    kp = 8 if len(P) > 256 else 4
+    sets = {}
+    for mu, n, l, m in P.tolist():
+        sets[(mu, l, m)] = sets.get((mu, l, m), 0) + 1
+    assert (sum(-(-v // kp) for v in sets.values()) <= 64) == shared
+    c = synth.coefficients(len(basis), 5)
+    off, R, _ = synth.workload_envs(2, nenv=70, seed=23)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), None, device="cuda:0")
+    E0, dE0 = ev.energy_forces(b)
+    E0, dE0 = E0.clone(), dE0.clone()
+    assert rel_err(E0.cpu().numpy(), O.energy(T, c, off, R)) < E_TOL
+    assert rel_err(dE0.cpu().numpy().T, O.grad_config(T, c, off, R)) < F_TOL
+    for ch in (8, 32):
+        ev.set_pool_shape(ch, 0)
+        E, dE = ev.energy_forces(b)
+        torch.cuda.synchronize()
+        assert torch.equal(E, E0) and torch.equal(dE, dE0), ch
 
 
 DB_CASES = [
