@@ -85,11 +85,21 @@ def executed_flops(pack, forces=True, kernel=0):
         w = pack.work_counts(forces)
         per_leaf = 8.0 if cplx else 4.0
         return per_leaf * w["leaves"] + 6.0 * w["prefix_mults"] + 8.0 * w["runs"]
+    d, nl, et = _segment_records(pack, forces)
+    lanes = 64.0 / et if kernel == 1 else 1.0
+    per_leaf = 8.0 if cplx else 4.0
+    rec = per_leaf * nl - per_leaf / 2 + 6.0 * np.maximum(d - 1, 0) + np.where(d > 0, 8.0, 2.0)
+    return float(lanes * np.sum(rec))
+
+
+def _segment_records(pack, forces):
+    """Prefix depth and leaf count of every segment-kernel record (each run's
+    leaves cut into records of at most 8, 4 above correlation order 4), and
+    the library's tile width."""
+    import numpy as np
     tasks = list(pack.energy_tasks) + (list(pack.grad_tasks) if forces else [])
     runs = np.concatenate([np.arange(pack.tasks[t][0], pack.tasks[t][1]) for t in tasks]) if tasks else []
     rd, rn = pack.runs[runs, 0].astype(np.int64), pack.runs[runs, 2].astype(np.int64)
-    # records: every run cut into records of at most 8 leaves (4 for
-    # correlation orders above 4)
     ch = 4 if pack.maxorder > 4 else 8
     nrec = (rn + ch - 1) // ch
     d = np.repeat(rd, nrec)
@@ -97,10 +107,20 @@ def executed_flops(pack, forces=True, kernel=0):
     q = np.arange(len(d)) - first                         # record index within its run
     nl = np.minimum(ch, np.repeat(rn, nrec) - ch * q)
     et = min(64, (160 * 1024) // (16 * len(pack.pool)))   # the library's tile width
+    return d, nl, et
+
+
+def lds_read_bytes(pack, forces=True, kernel=0):
+    """LDS bytes the segment kernel reads per environment: every record reads
+    its d prefix factors and s leaves, 16 bytes each per lane; on narrower
+    tiles (ET < 64) each environment costs 64 / ET lanes.  None for the other
+    kernels."""
+    import numpy as np
+    if kernel not in (0, 1):
+        return None
+    d, nl, et = _segment_records(pack, forces)
     lanes = 64.0 / et if kernel == 1 else 1.0
-    per_leaf = 8.0 if cplx else 4.0
-    rec = per_leaf * nl - per_leaf / 2 + 6.0 * np.maximum(d - 1, 0) + np.where(d > 0, 8.0, 2.0)
-    return float(lanes * np.sum(rec))
+    return float(16.0 * lanes * np.sum(d + nl))
 
 
 def dag_kernel_name(code):
@@ -363,6 +383,7 @@ def main():
     dag_parts, dag_tail, dag_kernel = ev.last_dag_shape()
     f_alg = algorithmic_flops(pk, True)
     f_exe = executed_flops(pk, True, dag_kernel)
+    lds_env = lds_read_bytes(pk, True, dag_kernel)
     dag_avg = float(np.mean(dag_ms))
     achieved = f_alg * b.nenv / (dag_avg * 1e-3) / 1e12
     achieved_exe = f_exe * b.nenv / (dag_avg * 1e-3) / 1e12
@@ -404,6 +425,9 @@ def main():
                      "flops_per_env_algorithmic": f_alg, "flops_per_env_executed": f_exe,
                      "executed_tflops": round(achieved_exe, 3),
                      "executed_frac": round(achieved_exe / FP64_PEAK_TFLOPS, 4),
+                     # the segment kernel's binding resource: 16-byte LDS reads
+                     "lds_read_bytes_per_env": lds_env,
+                     "lds_read_TBps": None if lds_env is None else round(lds_env * b.nenv / (dag_avg * 1e-3) / 1e12, 2),
                      "pool_ms": round(float(np.mean(pool_ms)), 4),
                      "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
                      "force_ms": round(float(np.mean(force_ms)), 4)},
