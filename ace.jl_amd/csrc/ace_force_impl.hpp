@@ -195,16 +195,20 @@ __global__ __launch_bounds__(64) void k_force_lds(RadialParams rp, int64_t nenv,
 }
 
 // k_force: one wave per environment, lanes are neighbours.  The
-// environment's adjoint is staged in LDS as a dense table
-// Gd[mu][l, m >= 0][n] (zero where the pooled function is absent), so the
-// contraction needs no index lookup.  R_n and dR_n stay in registers (nr <=
-// 16); the ALP values are produced m-outer, l-inner, by the same operations
-// as alp_eval_ed, so only the two previous degrees are live.
+// environment's adjoint is staged in LDS as a table Gd[mu][(l, m >= 0), n <
+// nrl[l]] (zero where the pooled function is absent; each (l, m) holds only
+// the n its degree admits, half the dense [l, m][nr] table at Lmax 5-6), so
+// the contraction needs no index lookup.  R_n and dR_n stay in registers (NRM
+// of them: 12 when nr <= 12, else 16); the ALP values are produced m-outer,
+// l-inner, by the same operations as alp_eval_ed, so only the two previous
+// degrees are live.
 struct ForceTab {
-    int8_t nrl[ACE_LMAX + 1];  // n bound per l (max pooled n over mu, m)
+    int8_t nrl[ACE_LMAX + 1];                            // n bound per l (max pooled n over mu, m)
+    int16_t off[(ACE_LMAX + 1) * (ACE_LMAX + 2) / 2];    // first entry of (l, m) within a species' table
+    int32_t spstride;                                    // entries per species: NC sum_l (l + 1) nrl[l]
 };
 
-template <int L, int PROP>
+template <int L, int PROP, int NRM>
 __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int64_t nenv,
                                                const int32_t* __restrict__ off,
                                                const double* __restrict__ xyz, int64_t nnbr,
@@ -218,17 +222,17 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
                                                double* __restrict__ dE) {
     constexpr int NP = (L + 1) * (L + 2) / 2;
     constexpr int NC = PROP == 0 ? 1 : 3;
-    extern __shared__ double2 Gd[];  // [nsp][NP][nr][NC]
+    extern __shared__ double2 Gd[];  // [nsp][spstride]
     const int lane = threadIdx.x;
     const int64_t env = xcd_env(blockIdx.x, nenv);
     const int nr = rp.nr;
     if (env >= nenv) return;
-    const int ntab = nsp * NP * nr * NC;
+    const int ntab = nsp * ft.spstride;
     for (int q = lane; q < ntab; q += 64) Gd[q] = make_double2(0.0, 0.0);
     __syncthreads();
     for (int k = lane; k < npool; k += 64) {
         const int mu = max(0, pool[4 * k]), n = pool[4 * k + 1] - 1, l = pool[4 * k + 2], m = pool[4 * k + 3];
-        const int d = ((mu * NP + idxp(l, m)) * nr + n) * NC;
+        const int d = mu * ft.spstride + ft.off[idxp(l, m)] + n * NC;
         double2 g[NC];
         gather_g<NC>(gslot_ptr, gslot_list, gcls, cls_u, Gbuf, estride, env, k, g);
 #pragma unroll
@@ -239,11 +243,11 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
     for (int j = j0 + lane; j < j1; j += 64) {
         const double x = xyz[j], y = xyz[nnbr + j], z = xyz[2 * nnbr + j];
         const int mu = species ? (int)species[j] : 0;
-        const double2* Gm = Gd + (int64_t)mu * NP * nr * NC;
+        const double2* Gm = Gd + (int64_t)mu * ft.spstride;
         const Sph S = cart2spher(x, y, z);
         double t, dtdr;
         transform_ed(rp, S.r, t, dtdr);
-        double R[FORCE_NRMAX], dR[FORCE_NRMAX];
+        double R[NRM], dR[NRM];
         {
             double p0 = rp.A[0] * fcut(rp, t), d0 = rp.A[0] * fcut_d(rp, t);
             R[0] = p0;
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
             R[1] = p1;
             dR[1] = d1;
 #pragma unroll
-            for (int n = 2; n < FORCE_NRMAX; ++n) {
+            for (int n = 2; n < NRM; ++n) {
                 if (n < nr) {
                     const double al = rp.A[n] * t + rp.B[n];
                     const double p2 = al * p1 + rp.C[n] * p0;
@@ -326,13 +330,13 @@ __global__ __launch_bounds__(64) void k_force(RadialParams rp, ForceTab ft, int6
                     Pb = Pl;
                     dPb = dPl;
                 }
-                const double2* Gl = Gm + idxp(l, m) * nr * NC;
+                const double2* Gl = Gm + ft.off[idxp(l, m)];
                 const int nb = ft.nrl[l];
                 double ax[NC], ay[NC], bx[NC], by[NC];
 #pragma unroll
                 for (int p = 0; p < NC; ++p) ax[p] = ay[p] = bx[p] = by[p] = 0.0;
 #pragma unroll
-                for (int n = 0; n < FORCE_NRMAX; ++n) {
+                for (int n = 0; n < NRM; ++n) {
                     if (n < nb) {
 #pragma unroll
                         for (int p = 0; p < NC; ++p) {
@@ -579,15 +583,34 @@ static void launch_force(ace_ctx* ctx, hipStream_t st, int64_t nenv, const int32
     }
     const int nsp = std::max(1, ctx->nspecies);
     const int32_t* gcls = ctx->classed ? ctx->gslot_cls.p : nullptr;
-    const size_t dense = (size_t)nsp * NP * ctx->nr * NC * sizeof(double2);
-    if (ctx->nr <= FORCE_NRMAX && dense <= 64 * 1024) {
-        ForceTab ft;
-        for (int l = 0; l <= ACE_LMAX; ++l) ft.nrl[l] = (int8_t)(l <= L ? ctx->nrl[l] : 0);
-        set_func_lds((const void*)k_force<L, PROP>, dense);
-        hipLaunchKernelGGL((k_force<L, PROP>), dim3((unsigned)nenv), dim3(64), dense, st, radial_params(ctx), ft,
-                           nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, nsp, ctx->pool.p,
-                           ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, gcls, ctx->cls_u.p, ws.Gbuf, estride,
-                           dE);
+    // the per-environment table: each (l, m >= 0) holds the n < nrl[l]
+    ForceTab ft;
+    std::memset(&ft, 0, sizeof(ft));
+    int ent = 0;
+    for (int l = 0; l <= ACE_LMAX; ++l) {
+        ft.nrl[l] = (int8_t)(l <= L ? ctx->nrl[l] : 0);
+        for (int m = 0; m <= l && l <= L; ++m) {
+            ft.off[idxp(l, m)] = (int16_t)ent;
+            ent += ctx->nrl[l] * NC;
+        }
+    }
+    ft.spstride = ent;
+    const size_t table = (size_t)nsp * ent * sizeof(double2);
+    if (ctx->nr <= FORCE_NRMAX && table <= 64 * 1024) {
+#define ACE_FORCE(NRM_)                                                                                          \
+    do {                                                                                                          \
+        set_func_lds((const void*)k_force<L, PROP, NRM_>, table);                                                 \
+        hipLaunchKernelGGL((k_force<L, PROP, NRM_>), dim3((unsigned)nenv), dim3(64), table, st, radial_params(ctx), \
+                           ft, nenv, off, xyz, nnbr, ctx->nspecies > 0 ? species : nullptr, nsp, ctx->pool.p,     \
+                           ctx->npool, ctx->gslot_ptr.p, ctx->gslot_list.p, gcls, ctx->cls_u.p, ws.Gbuf, estride, \
+                           dE);                                                                                   \
+    } while (0)
+        // radial functions held in registers: 12 when they suffice, else 16
+        if (ctx->nr <= 12)
+            ACE_FORCE(12);
+        else
+            ACE_FORCE(FORCE_NRMAX);
+#undef ACE_FORCE
         return;
     }
     const size_t lds = (size_t)ctx->npool * NC * sizeof(double2) + 64 * 2 * ctx->nr * sizeof(double);
