@@ -466,6 +466,38 @@ def test_pool_lane_sets_large_bases(cuda, args, shared):
         assert torch.equal(E, E0) and torch.equal(dE, dE0), ch
 
 
+
+FORCE_TABLE_CASES = [
+    ("two-species-nr14", (2, 14, 4), {"species": 2}),        # 16 radial registers, Invariant with species
+    ("two-species-nr10", (2, 10, 4), {"species": 2}),        # 12 radial registers
+    ("vector-nr13", (3, 13, 3), {"prop": "EuclideanVector"}),  # 16 radial registers, three components
+]
+
+
+@pytest.mark.parametrize("name,args,kw", FORCE_TABLE_CASES, ids=[c[0] for c in FORCE_TABLE_CASES])
+def test_force_table_radial_registers(cuda, name, args, kw):
+    """The per-environment force kernel (species or vector property) with its
+    compact adjoint table (each (l, m) holds only the n its degree admits) and
+    12 or 16 radial functions in registers: energies and every per-neighbour
+    gradient against the oracle."""
+    print(f"[gpu] per-environment force table: {name}", flush=True)
+    basis, T = basis_for(*args, **kw)
+    c = synth.coefficients(len(basis), 6)
+    ns = kw.get("species", 0)
+    off, R, sp = random_batch(40, 90 + ns, nmin=3, nmax=40, nspecies=ns)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    E, dE = ev.energy_forces(b)
+    torch.cuda.synchronize()
+    E, dE = E.cpu().numpy(), dE.cpu().numpy()
+    assert rel_err(E, O.energy(T, c, off, R, sp)) < E_TOL
+    g_ref = O.grad_config(T, c, off, R, sp)
+    if kw.get("prop") == "EuclideanVector":
+        assert rel_err(dE.reshape(3, 3, -1).transpose(2, 0, 1), g_ref) < F_TOL
+    else:
+        assert rel_err(dE.T, g_ref) < F_TOL
+
+
 DB_CASES = [
     ("config2", (4, 12, 6), {}, 3),                                            # 3235 B: 13 chunks of 256
     ("two-species", (3, 6, 3), {"species": 2}, 6),
