@@ -183,28 +183,46 @@ __global__ void k_AAdir(int64_t nenv, int nA, int nAA, int maxord, const int32_t
 
 
 
-// Product (DIR = false) or dual-number derivative (DIR = true) of the AA
-// column of record r, left to right, from the staged tiles.
-template <bool DIR, int ENVS>
-__device__ __forceinline__ double2 bt_product(const double2* As, const double2* Ws, const BtRec& r, int e) {
-    double2 a[4], da[4];
+// Factor f of the staged tile, conjugated when f carries ACE_CONJ (the sign
+// bit of the imaginary part flipped without a branch).
+template <int ENVS>
+__device__ __forceinline__ double2 bt_factor(const double2* As, int f, int e) {
+    const double2 v = As[(f & 0x7fff) * ENVS + e];
+    return make_double2(v.x, __hiloint2double(__double2hiint(v.y) ^ ((f & ACE_CONJ) << 16), __double2loint(v.y)));
+}
+
+// The records [za, zb) of one row whose AA columns have order O: each
+// recomputes its product (or dual-number derivative, DIR) left to right
+// from the staged tiles, straight-line code, four records per iteration so
+// that their scalar loads and LDS reads overlap (two for the dual numbers of
+// DIR: registers).
+template <int O, int PROP, bool DIR, int ENVS>
+__device__ __forceinline__ void bt_order(int za, int zb, const BtRec* __restrict__ recs,
+                                         const double* __restrict__ vals, const double2* As, const double2* Ws,
+                                         int e, double* s) {
+#pragma unroll(DIR ? 2 : 4)
+    for (int z = za; z < zb; ++z) {
+        const BtRec r = recs[z];
+        double2 p = make_double2(1.0, 0.0), dp = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (q < r.o) {
-            a[q] = lds_factor<ENVS>(As, r.f[q], e);
-            if (DIR) da[q] = lds_factor<ENVS>(Ws, r.f[q], e);
-        }
-    double2 p = make_double2(1.0, 0.0), dp = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (q < r.o) {
+        for (int q = 0; q < O; ++q) {
+            const double2 a = bt_factor<ENVS>(As, r.f[q], e);
             if (DIR) {
-                const double2 x = cmul(dp, a[q]), y = cmul(p, da[q]);
+                const double2 da = bt_factor<ENVS>(Ws, r.f[q], e);
+                const double2 x = cmul(dp, a), y = cmul(p, da);
                 dp = make_double2(x.x + y.x, x.y + y.y);
             }
-            p = cmul(p, a[q]);
+            p = cmul(p, a);
         }
-    return DIR ? dp : p;
+        const double2 v = DIR ? dp : p;
+        if (PROP == 0) {
+            s[0] += r.coef * v.x;
+        } else {
+            const double sg = r.coef;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) s[q] += sg * (vals[6 * r.z + 2 * q] * v.x - vals[6 * r.z + 2 * q + 1] * v.y);
+        }
+    }
 }
 
 // k_Bt: basis values B = A2B . AA (DIR = false; pibasis.jl:265-275,
@@ -212,16 +230,17 @@ __device__ __forceinline__ double2 bt_product(const double2* As, const double2* 
 // per-neighbour weights (DIR = true; evaluator.jl:225-240) for a tile of
 // ENVS environments, without materialising AA.  The pooled A+ tile (and
 // dA+w for DIR) is staged in LDS as in k_dagr; every A2B nonzero (one BtRec,
-// read with one scalar load) recomputes its AA product from it.  A
-// workgroup is (tile, chunk of rows); each wave takes rpw consecutive rows
-// (with ENVS = 32 the two half-waves take alternate rows).  Workgroups of
-// one tile are consecutive on one XCD so the staged tile is read from that
-// XCD's L2.
+// read with scalar loads) recomputes its AA product from it.  A row's
+// nonzeros are ordered by the order of their AA column (rowo: where each
+// order starts), one straight-line loop per order.  A workgroup is (tile,
+// chunk of rows); each wave takes rpw consecutive rows (with ENVS = 32 the two
+// half-waves take alternate rows).  Workgroups of one tile are consecutive on
+// one XCD so the staged tile is read from that XCD's L2.
 constexpr int BT_NW = 16;  // waves per k_Bt workgroup
 
 template <int PROP, bool DIR, int ENVS>
 __global__ __launch_bounds__(64 * BT_NW) void k_Bt(int64_t nenv, int npool, int nB, int nchunk, int rpw,
-                                            const int32_t* __restrict__ rowptr, const BtRec* __restrict__ recs,
+                                            const int32_t* __restrict__ rowo, const BtRec* __restrict__ recs,
                                             const double* __restrict__ vals, const double2* __restrict__ Aplus,
                                             const double2* __restrict__ Wplus, int64_t estride,
                                             double* __restrict__ out) {
@@ -257,19 +276,12 @@ __global__ __launch_bounds__(64 * BT_NW) void k_Bt(int64_t nenv, int npool, int 
         double s[NC];
 #pragma unroll
         for (int q = 0; q < NC; ++q) s[q] = 0.0;
-        const int z1 = rowptr[b + 1];
-        for (int z = rowptr[b]; z < z1; ++z) {
-            const BtRec r = recs[z];
-            const double2 v = bt_product<DIR, ENVS>(As, Ws, r, e);
-            if (PROP == 0) {
-                s[0] += r.coef * v.x;
-            } else {
-                const double sg = r.coef;
-#pragma unroll
-                for (int q = 0; q < NC; ++q)
-                    s[q] += sg * (vals[6 * r.z + 2 * q] * v.x - vals[6 * r.z + 2 * q + 1] * v.y);
-            }
-        }
+        const int32_t* ro = rowo + (int64_t)b * BT_ROWO;
+        bt_order<0, PROP, DIR, ENVS>(ro[0], ro[1], recs, vals, As, Ws, e, s);
+        bt_order<1, PROP, DIR, ENVS>(ro[1], ro[2], recs, vals, As, Ws, e, s);
+        bt_order<2, PROP, DIR, ENVS>(ro[2], ro[3], recs, vals, As, Ws, e, s);
+        bt_order<3, PROP, DIR, ENVS>(ro[3], ro[4], recs, vals, As, Ws, e, s);
+        bt_order<4, PROP, DIR, ENVS>(ro[4], ro[5], recs, vals, As, Ws, e, s);
         if (env < nenv) {
 #pragma unroll
             for (int q = 0; q < NC; ++q) out[(env * nB + b) * NC + q] = s[q];
@@ -583,13 +595,13 @@ static int launch_bt(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estride
     if (ctx->prop == ACE_PROP_INVARIANT) {
         set_func_lds((const void*)k_Bt<0, DIR, ENVS>, lds);
         hipLaunchKernelGGL((k_Bt<0, DIR, ENVS>), dim3((unsigned)nblk), dim3(64 * NW), lds, st, nenv, ctx->npool,
-                           ctx->nB, nchunk, rpw, ctx->rowptr.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
+                           ctx->nB, nchunk, rpw, ctx->bt_rowo.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
                            ctx->Wplus.p, estride, out);
     } else {
         if (DIR) return set_err("directional basis derivative: Invariant bases only");
         set_func_lds((const void*)k_Bt<1, false, 64>, lds);
         hipLaunchKernelGGL((k_Bt<1, false, 64>), dim3((unsigned)nblk), dim3(64 * NW), lds, st, nenv, ctx->npool,
-                           ctx->nB, nchunk, rpw, ctx->rowptr.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
+                           ctx->nB, nchunk, rpw, ctx->bt_rowo.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
                            ctx->Wplus.p, estride, out);
     }
     HIP_TRY(hipGetLastError());

@@ -218,6 +218,9 @@ struct BtRec {
     double coef;
 };
 static_assert(sizeof(BtRec) == 32, "BtRec is 8 dwords");
+// per basis row of the fused kernel: where its records of order 0, 1, ..., 4
+// start, and the row's end
+constexpr int BT_ROWO = 6;
 
 // ---------------------------------------------------------------------------
 // host side
@@ -291,6 +294,7 @@ struct ace_ctx {
     DevBuf<int32_t> pool, aspec_fid, aspec4, orders, iAA2iA, rowptr, cols;
     DevBuf<double> aspec_sgn, vals;
     DevBuf<BtRec> bt_rec;
+    DevBuf<int32_t> bt_rowo;  // [nB][BT_ROWO] record ranges per order
     DevBuf<int32_t> ffq;  // G slots of the flat force kernel's staged (l, m, n) entries
     DevBuf<int16_t> kidx;
     int nrl[ACE_LMAX + 1] = {0};  // n bound per l over the pooled functions

@@ -940,7 +940,24 @@ extern "C" int ace_ctx_create(const ace_pack* pk, int device, const double* c, i
             }
             r.coef = pk->prop == ACE_PROP_INVARIANT ? pk->a2b_vals[z] * sg : sg;
         }
-        if (ctx->bt_rec.upload(rec.data(), rec.size())) return fail("upload");
+        // every row's nonzeros ordered by the order of their AA column, and
+        // where each order's records start, so that the kernel runs one
+        // straight-line loop per order (orders 0 to 4; larger bases do not
+        // use the fused kernel)
+        std::vector<int32_t> rowo((size_t)std::max(1, pk->nB) * BT_ROWO, 0);
+        for (int b = 0; b < pk->nB; ++b) {
+            const int z0 = pk->a2b_rowptr[b], z1 = pk->a2b_rowptr[b + 1];
+            std::stable_sort(rec.begin() + z0, rec.begin() + z1, [](const BtRec& x, const BtRec& y) { return x.o < y.o; });
+            int32_t* ro = rowo.data() + (size_t)b * BT_ROWO;
+            ro[0] = z0;
+            for (int o = 0; o + 1 < BT_ROWO; ++o) {
+                int n = 0;
+                for (int z = z0; z < z1; ++z) n += rec[z].o == o;
+                ro[o + 1] = ro[o] + n;
+            }
+        }
+        if (ctx->bt_rec.upload(rec.data(), rec.size()) || ctx->bt_rowo.upload(rowo.data(), rowo.size()))
+            return fail("upload");
         std::vector<int32_t> a4 = acehost::aspec_rows(pk);
         if (ctx->aspec4.upload(a4.data(), a4.size())) return fail("upload");
         std::vector<int16_t> kx = acehost::pool_lookup(pk);
