@@ -576,36 +576,46 @@ extern "C" int ace_basis_AA(ace_ctx* ctx, int64_t nenv, const int32_t* off, cons
     return 0;
 }
 
-// Fused basis values (k_Bt): whether the pooled tile fits LDS, and the launch.
-static bool bt_fits(const ace_ctx* ctx, bool dir) {
-    const int envs = dir ? 32 : 64;
-    if (ctx->maxorder > 4) return false;  // BtRec holds four factors
-    return (size_t)ctx->npool * envs * sizeof(double2) * (dir ? 2 : 1) <= 160 * 1024;
+// Fused basis values (k_Bt): the tile it runs on (environments; 0 when the
+// pooled tile does not fit LDS), and the launch.  Basis values take 64
+// environments when their A+ tile fits, else 32 (npool <= 320); the
+// directional derivative stages A+ and dA+w, 32 environments.
+static int bt_envs(const ace_ctx* ctx, bool dir) {
+    if (ctx->maxorder > 4) return 0;  // BtRec holds four factors
+    const size_t row = (size_t)ctx->npool * sizeof(double2) * (dir ? 2 : 1);
+    if (!dir && row * 64 <= 160 * 1024) return 64;
+    return row * 32 <= 160 * 1024 ? 32 : 0;
 }
 
-template <bool DIR>
-static int launch_bt(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estride, double* out) {
-    constexpr int ENVS = DIR ? 32 : 64;
+template <int PROP, bool DIR, int ENVS>
+static int launch_bt_shape(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estride, double* out) {
     const size_t lds = (size_t)ctx->npool * ENVS * sizeof(double2) * (DIR ? 2 : 1);
     constexpr int NW = BT_NW;
     const int rpw = std::max(1, (ctx->nB + NW * 12 - 1) / (NW * 12));
     const int nchunk = (ctx->nB + NW * rpw - 1) / (NW * rpw);
     const int64_t nblk = (nenv + ENVS - 1) / ENVS * nchunk;
     if (nblk >= ((int64_t)1 << 31)) return set_err("batch too large for one launch");
-    if (ctx->prop == ACE_PROP_INVARIANT) {
-        set_func_lds((const void*)k_Bt<0, DIR, ENVS>, lds);
-        hipLaunchKernelGGL((k_Bt<0, DIR, ENVS>), dim3((unsigned)nblk), dim3(64 * NW), lds, st, nenv, ctx->npool,
-                           ctx->nB, nchunk, rpw, ctx->bt_rowo.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
-                           ctx->Wplus.p, estride, out);
-    } else {
-        if (DIR) return set_err("directional basis derivative: Invariant bases only");
-        set_func_lds((const void*)k_Bt<1, false, 64>, lds);
-        hipLaunchKernelGGL((k_Bt<1, false, 64>), dim3((unsigned)nblk), dim3(64 * NW), lds, st, nenv, ctx->npool,
-                           ctx->nB, nchunk, rpw, ctx->bt_rowo.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p,
-                           ctx->Wplus.p, estride, out);
-    }
+    set_func_lds((const void*)k_Bt<PROP, DIR, ENVS>, lds);
+    hipLaunchKernelGGL((k_Bt<PROP, DIR, ENVS>), dim3((unsigned)nblk), dim3(64 * NW), lds, st, nenv, ctx->npool,
+                       ctx->nB, nchunk, rpw, ctx->bt_rowo.p, ctx->bt_rec.p, ctx->vals.p, ctx->Aplus.p, ctx->Wplus.p,
+                       estride, out);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+template <bool DIR>
+static int launch_bt(ace_ctx* ctx, hipStream_t st, int64_t nenv, int64_t estride, double* out) {
+    const int envs = bt_envs(ctx, DIR);
+    if (envs == 0) return set_err("pooled tile does not fit LDS");
+    if (DIR) {
+        if (ctx->prop != ACE_PROP_INVARIANT) return set_err("directional basis derivative: Invariant bases only");
+        return launch_bt_shape<0, true, 32>(ctx, st, nenv, estride, out);
+    }
+    if (ctx->prop == ACE_PROP_INVARIANT)
+        return envs == 64 ? launch_bt_shape<0, false, 64>(ctx, st, nenv, estride, out)
+                          : launch_bt_shape<0, false, 32>(ctx, st, nenv, estride, out);
+    return envs == 64 ? launch_bt_shape<1, false, 64>(ctx, st, nenv, estride, out)
+                      : launch_bt_shape<1, false, 32>(ctx, st, nenv, estride, out);
 }
 
 extern "C" int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* off, const double* xyz,
@@ -617,8 +627,7 @@ extern "C" int ace_basis_B(ace_ctx* ctx, int64_t nenv, const int32_t* off, const
     hipStream_t st = (hipStream_t)stream;
     {
         const int64_t estride = ((nenv + 63) / 64) * 64;
-        const int fits = bt_fits(ctx, false);
-        if (fits) {
+        if (bt_envs(ctx, false) > 0) {
             if (ctx->Aplus.ensure((size_t)ctx->npool * estride)) return -1;
             if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), false)) return -1;
             return launch_bt<false>(ctx, st, nenv, estride, B_out);
@@ -756,7 +765,7 @@ extern "C" int ace_adjoint_EVAL_D(ace_ctx* ctx, int64_t nenv, const int32_t* off
     if (!dB_out || (nnbr > 0 && !w)) return set_err("ace_adjoint_EVAL_D: null argument");
     hipStream_t st = (hipStream_t)stream;
     const int64_t estride = ((nenv + 63) / 64) * 64;
-    if (bt_fits(ctx, true)) {
+    if (bt_envs(ctx, true) > 0) {
         if (ctx->Aplus.ensure((size_t)ctx->npool * estride) || ctx->Wplus.ensure((size_t)ctx->npool * estride))
             return -1;
         if (launch_pool_any(ctx, st, nenv, off, xyz, nnbr, species, estride, ctx_buffers(ctx), false) ||

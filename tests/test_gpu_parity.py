@@ -314,6 +314,27 @@ def test_basis_level(cuda):
     assert rel_err(dB, O.eval_dB(T, dAA)) < F_TOL
 
 
+
+@pytest.mark.parametrize("args,kw", [((3, 10, 5), {"prop": "EuclideanVector", "species": 2}), ((2, 16, 6), {})],
+                         ids=["vector-two-species-npool220", "npool251"])
+def test_basis_B_32_environment_tiles(cuda, args, kw):
+    """Basis values of bases whose A+ tile of 64 environments exceeds the LDS
+    (more than 160 pooled values) run the fused kernel on 32-environment
+    tiles, the two half-waves taking alternate rows; over three tiles and a
+    partial one, against the oracle's B = A2B . AA."""
+    print(f"[gpu] basis B on 32-environment tiles {args} {kw}", flush=True)
+    basis, T = basis_for(*args, **kw)
+    c = synth.coefficients(len(basis), 2)
+    ns = kw.get("species", 0)
+    off, R, sp = random_batch(100, 41, nmin=3, nmax=30, nspecies=ns)
+    ev = HIPEvaluator(basis, c, device=0)
+    b = EnvBatch(off, R.T.copy(), sp, device="cuda:0")
+    B = ev.basis_B(b).cpu().numpy()
+    ref = O.eval_B(T, O.eval_AA(T, O.eval_A(T, off, R, sp)))
+    assert B.size == ref.size
+    assert rel_err(B.reshape(ref.shape), ref) < E_TOL
+
+
 @pytest.mark.parametrize("nenv,species", [(1, 0), (3, 0), (65, 0), (33, 2)])
 def test_two_environments_per_wave(cuda, nenv, species):
     """Bases with at most 32 pooled values pool two environments per wave;
